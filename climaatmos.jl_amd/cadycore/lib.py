@@ -1,0 +1,252 @@
+"""Python host side of the native dry-dycore step.
+
+Binds ``libcadycore.so`` (C ABI in ``include/cadycore.h``) with ctypes and
+mirrors the reference's tendency interface (``src/simulation/integrator.jl:
+193-225``): ``remaining_tendency``, ``implicit_tendency``,
+``update_jacobian`` (Wfact), ``ldiv``, ``correct_implicit_advection_tendency``
+(T_post_imp!), ``set_implicit_precomputed_quantities`` (cache_imp!), ``dss``
+and ``step`` (one ARS343 step of ``CTS.step!``).  States are float32 torch
+tensors on the GPU in the native flat layout ``[rho | u1 | u2 | rho_e | u3]``.
+
+There is no CPU fallback: constructing a :class:`DryDycore` without the
+built library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .geometry import Grid, gll
+from .params import Numerics, Params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcadycore.so")
+
+CA_OK = 0
+_STATUS = {0: "CA_OK", 1: "CA_ERR_ARG", 2: "CA_ERR_DEVICE", 3: "CA_ERR_NONFINITE"}
+
+
+class CaGeometry(C.Structure):
+    _fields_ = [
+        ("nelem", C.c_int32), ("ncol", C.c_int32), ("nlev", C.c_int32), ("radius", C.c_double),
+        ("z_face", C.POINTER(C.c_double)), ("J_h", C.POINTER(C.c_double)),
+        ("gup", C.POINTER(C.c_double)), ("gdn", C.POINTER(C.c_double)),
+        ("f3_num", C.POINTER(C.c_double)), ("f12", C.POINTER(C.c_double)),
+        ("basis", C.POINTER(C.c_double)), ("n_dss", C.c_int32),
+        ("dss_ptr", C.POINTER(C.c_int32)), ("dss_col", C.POINTER(C.c_int32)),
+        ("dss_w", C.POINTER(C.c_double)), ("gll_w", C.POINTER(C.c_double)),
+        ("gll_D", C.POINTER(C.c_double)),
+    ]
+
+
+class CaParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("R_d", "cp_d", "cv_d", "T_0", "p_ref_theta", "grav", "T_surf_ref", "T_min_ref", "T_min_sgs")]
+
+
+class CaConfig(C.Structure):
+    _fields_ = [
+        ("hyperdiff", C.c_int32), ("nu4_vort", C.c_double), ("nu4_scal", C.c_double),
+        ("div_damp", C.c_double), ("upwinding", C.c_int32), ("rayleigh", C.c_int32),
+        ("zd_rayleigh", C.c_double), ("alpha_uh", C.c_double), ("alpha_w", C.c_double),
+        ("alpha_w_col", C.POINTER(C.c_double)), ("vert_diff", C.c_int32),
+        ("D0_col", C.POINTER(C.c_double)), ("dt", C.c_double),
+    ]
+
+
+class CaLayout(C.Structure):
+    _fields_ = [("offset", C.c_int64), ("s_v", C.c_int64), ("s_i", C.c_int64),
+                ("s_j", C.c_int64), ("s_h", C.c_int64)]
+
+
+EXPORTED = (
+    "ca_create", "ca_destroy", "ca_state_size", "ca_last_error", "ca_set_precomputed",
+    "ca_remaining_tendency", "ca_implicit_tendency", "ca_update_jacobian", "ca_ldiv",
+    "ca_correct_implicit_advection", "ca_dss", "ca_step", "ca_pack_state", "ca_unpack_state",
+    "ca_check_finite", "ca_profile_enable", "ca_profile_count", "ca_profile_get",
+)
+
+
+def load_library(path: str = LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"libcadycore.so not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.ca_create.argtypes = [C.POINTER(CaGeometry), C.POINTER(CaParams), C.POINTER(CaConfig), C.POINTER(vp)]
+    lib.ca_destroy.argtypes = [vp]
+    lib.ca_state_size.argtypes = [vp]
+    lib.ca_state_size.restype = C.c_int64
+    lib.ca_last_error.argtypes = [vp]
+    lib.ca_last_error.restype = C.c_char_p
+    for nm in ("ca_set_precomputed", "ca_dss"):
+        getattr(lib, nm).argtypes = [vp, vp, vp]
+    for nm in ("ca_remaining_tendency", "ca_implicit_tendency", "ca_ldiv", "ca_correct_implicit_advection"):
+        getattr(lib, nm).argtypes = [vp, vp, vp, vp]
+    lib.ca_update_jacobian.argtypes = [vp, vp, C.c_double, vp]
+    lib.ca_step.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
+    lib.ca_pack_state.argtypes = [vp, vp, C.POINTER(CaLayout), vp, vp]
+    lib.ca_unpack_state.argtypes = [vp, vp, C.POINTER(CaLayout), vp, vp]
+    lib.ca_check_finite.argtypes = [vp, vp, C.POINTER(C.c_int32), vp]
+    lib.ca_profile_enable.argtypes = [vp, C.c_int32]
+    lib.ca_profile_count.argtypes = []
+    lib.ca_profile_get.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double)]
+    for nm in EXPORTED:
+        if nm not in ("ca_state_size", "ca_last_error", "ca_profile_count"):
+            getattr(lib, nm).restype = C.c_int
+    lib.ca_profile_count.restype = C.c_int32
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DryDycore:
+    """One native dry-dycore instance on the current GPU.
+
+    ``grid``/``params``/``numerics`` describe the configuration;
+    ``alpha_w_col`` and ``D0_col`` give per-column Rayleigh sponge strengths
+    and vertical diffusivities (ensembles and calibration).
+    """
+
+    def __init__(self, grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None, D0_col=None,
+                 device=None):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("DryDycore needs a GPU (no CPU fallback)")
+        self.torch = torch
+        self.device = torch.device(device or "cuda:0")
+        torch.cuda.set_device(self.device)
+        self.lib = load_library()
+        self.grid, self.params, self.numerics = grid, params, numerics
+        keep = {}
+
+        def arr(name, a, dt=np.float64):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep[name] = a
+            return a
+
+        _, wq, D = gll()
+        g = CaGeometry()
+        g.nelem, g.ncol, g.nlev, g.radius = grid.nelem, grid.ncol, grid.nlev, grid.radius
+        g.z_face = _dp(arr("z", grid.z_face))
+        g.J_h = _dp(arr("J", grid.J_h))
+        g.gup = _dp(arr("gup", grid.gup))
+        g.gdn = _dp(arr("gdn", grid.gdn))
+        g.f3_num = _dp(arr("f3", grid.f3_num))
+        g.f12 = _dp(arr("f12", grid.f12))
+        g.basis = _dp(arr("b", grid.basis))
+        g.n_dss = len(grid.dss_ptr) - 1
+        g.dss_ptr = arr("dp", grid.dss_ptr, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+        g.dss_col = arr("dc", grid.dss_col, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+        g.dss_w = _dp(arr("dw", grid.dss_w))
+        g.gll_w = _dp(arr("w", wq))
+        g.gll_D = _dp(arr("D", D))
+        p = CaParams(params.R_d, params.cp_d, params.cv_d, params.T_0, params.p_ref_theta, params.grav,
+                     params.T_surf_ref, params.T_min_ref, params.T_min_sgs)
+        c = CaConfig()
+        c.hyperdiff = int(bool(numerics.hyperdiff) and grid.nelem > 0)
+        nu_v, nu_s = numerics.nu4(grid.h) if grid.nelem > 0 else (0.0, 0.0)
+        c.nu4_vort, c.nu4_scal, c.div_damp = nu_v, nu_s, numerics.div_damping
+        c.upwinding = int(numerics.upwinding)
+        c.rayleigh = int(bool(numerics.rayleigh))
+        c.zd_rayleigh, c.alpha_uh, c.alpha_w = params.zd_rayleigh, params.alpha_rayleigh_uh, params.alpha_rayleigh_w
+        if alpha_w_col is not None:
+            c.alpha_w_col = _dp(arr("aw", alpha_w_col))
+        c.vert_diff = int(bool(numerics.vert_diff))
+        if numerics.vert_diff:
+            if D0_col is None:
+                raise ValueError("vertical diffusion needs D0_col")
+            c.D0_col = _dp(arr("D0", D0_col))
+        c.dt = numerics.dt
+        h = C.c_void_p()
+        st = self.lib.ca_create(C.byref(g), C.byref(p), C.byref(c), C.byref(h))
+        if st != CA_OK:
+            raise RuntimeError(f"ca_create failed: {_STATUS.get(st, st)}")
+        self.h = h
+        self.S = int(self.lib.ca_state_size(h))
+        self._keep = keep
+
+    # ------------------------------------------------------------ plumbing
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ca_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _ptr(self, t):
+        assert t.is_cuda and t.dtype == self.torch.float32 and t.is_contiguous() and t.numel() == self.S
+        return C.c_void_p(t.data_ptr())
+
+    def _check(self, st, what):
+        if st != CA_OK:
+            msg = self.lib.ca_last_error(self.h)
+            raise RuntimeError(f"{what}: {_STATUS.get(st, st)}: {msg.decode() if msg else ''}")
+
+    def state(self, Y_np):
+        """Upload a numpy state (float64 or float32) as a device tensor."""
+        return self.torch.tensor(np.asarray(Y_np, dtype=np.float32), device=self.device)
+
+    def zeros(self):
+        return self.torch.zeros(self.S, dtype=self.torch.float32, device=self.device)
+
+    # ------------------------------------------------- reference interface
+    def set_implicit_precomputed_quantities(self, Y):
+        self._check(self.lib.ca_set_precomputed(self.h, self._ptr(Y), self._stream()), "cache_imp!")
+
+    def remaining_tendency(self, Yt, Y):
+        self._check(self.lib.ca_remaining_tendency(self.h, self._ptr(Yt), self._ptr(Y), self._stream()),
+                    "remaining_tendency!")
+        return Yt
+
+    def implicit_tendency(self, Yt, Y):
+        self._check(self.lib.ca_implicit_tendency(self.h, self._ptr(Yt), self._ptr(Y), self._stream()),
+                    "implicit_tendency!")
+
+    def update_jacobian(self, Y, dtgamma):
+        self._check(self.lib.ca_update_jacobian(self.h, self._ptr(Y), float(dtgamma), self._stream()),
+                    "update_jacobian!")
+
+    def ldiv(self, dY, R):
+        self._check(self.lib.ca_ldiv(self.h, self._ptr(dY), self._ptr(R), self._stream()), "ldiv!")
+
+    def correct_implicit_advection_tendency(self, Yt, Y):
+        self._check(self.lib.ca_correct_implicit_advection(self.h, self._ptr(Yt), self._ptr(Y), self._stream()),
+                    "correct_implicit_advection_tendency!")
+
+    def dss(self, Y):
+        self._check(self.lib.ca_dss(self.h, self._ptr(Y), self._stream()), "dss!")
+
+    def step(self, Y, dt, nsteps=1):
+        self._check(self.lib.ca_step(self.h, self._ptr(Y), float(dt), int(nsteps), self._stream()), "step!")
+
+    def check_finite(self, Y):
+        flag = C.c_int32(0)
+        st = self.lib.ca_check_finite(self.h, self._ptr(Y), C.byref(flag), self._stream())
+        return st == CA_OK and flag.value == 0
+
+    # ------------------------------------------------------------ profiling
+    def profile_enable(self, on=True):
+        self._check(self.lib.ca_profile_enable(self.h, int(on)), "profile_enable")
+
+    def profile(self):
+        out = {}
+        for i in range(self.lib.ca_profile_count()):
+            name = C.c_char_p()
+            n = C.c_int64()
+            ms = C.c_double()
+            self._check(self.lib.ca_profile_get(self.h, i, C.byref(name), C.byref(n), C.byref(ms)), "profile_get")
+            out[name.value.decode()] = (int(n.value), float(ms.value))
+        return out

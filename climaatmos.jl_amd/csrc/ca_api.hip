@@ -1,0 +1,615 @@
+// ca_api.hip -- host side of the C ABI declared in include/cadycore.h:
+// handle life cycle, geometry upload, the hook-level entry points and the
+// fused ARS343 step driver.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cadycore.h"
+#include "ca_kernels.h"
+
+
+namespace {
+
+enum KernelId {
+    KID_LAPLACIANS = 0,
+    KID_EXPLICIT,
+    KID_DSS_TENDENCY,
+    KID_DSS_LAPLACIAN,
+    KID_NEWTON_STAGE,
+    KID_FINAL,
+    KID_DSS_STATE,
+    KID_HOOKS,
+    KID_COUNT
+};
+const char *kKernelNames[KID_COUNT] = {"k_laplacians", "k_explicit", "k_dss(tendency)", "k_dss(laplacians)",
+                                       "k_newton_stage", "k_final", "k_dss(state)", "hook kernels"};
+
+struct ProfRec {
+    int id;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ca_handle {
+    int device = 0;
+    DevGeom g{};
+    DevNum n{};
+    int64_t S = 0;
+    std::vector<void *> allocs;
+    float *pre = nullptr;  // K, T, p, h planes
+    float *L = nullptr;    // 4 Laplacian planes
+    float *scr = nullptr;  // column scratch: 2 face planes + 6 centre planes
+    float *FE[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *FI[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *U = nullptr;
+    float *bands = nullptr;
+    Bands B{};
+    int *flag = nullptr;
+    std::string err;
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    double aE[4][4], aI[4][4], b[4];
+};
+
+namespace {
+
+ca_status fail(ca_handle *h, ca_status s, const std::string &msg) {
+    if (h) h->err = msg;
+    return s;
+}
+
+#define HIPCHK(h, call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(h, CA_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+ca_status dalloc(ca_handle *h, T **p, size_t count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 16);
+    if (e != hipSuccess) return fail(h, CA_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = (T *)q;
+    return CA_OK;
+}
+
+template <class T>
+ca_status upload(ca_handle *h, T **dst, const std::vector<T> &src) {
+    ca_status s = dalloc(h, dst, src.size() ? src.size() : 1);
+    if (s) return s;
+    if (!src.empty()) HIPCHK(h, hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return CA_OK;
+}
+
+// ARS(3,4,3) (Ascher, Ruuth & Spiteri 1997) with the a42 = a43 explicit family
+void ars343(ca_handle *h) {
+    double g = 0.4358665215;
+    for (int it = 0; it < 50; ++it) g -= (6 * g * g * g - 18 * g * g + 9 * g - 1) / (18 * g * g - 36 * g + 9);
+    double b1 = -1.5 * g * g + 4 * g - 0.25, b2 = 1.5 * g * g - 5 * g + 1.25;
+    double a42 = 0.5529291480359398, a43 = a42;
+    double a31 = (1 - 4.5 * g + 1.5 * g * g) * a42 + (2.75 - 10.5 * g + 3.75 * g * g) * a43 - 3.5 + 13 * g - 4.5 * g * g;
+    double a32 = (-1 + 4.5 * g - 1.5 * g * g) * a42 + (-2.75 + 10.5 * g - 3.75 * g * g) * a43 + 4 - 12.5 * g + 4.5 * g * g;
+    memset(h->aE, 0, sizeof(h->aE));
+    memset(h->aI, 0, sizeof(h->aI));
+    h->aE[1][0] = g;
+    h->aE[2][0] = a31;
+    h->aE[2][1] = a32;
+    h->aE[3][0] = 1 - a42 - a43;
+    h->aE[3][1] = a42;
+    h->aE[3][2] = a43;
+    h->aI[1][1] = g;
+    h->aI[2][1] = (1 - g) / 2;
+    h->aI[2][2] = g;
+    h->aI[3][1] = b1;
+    h->aI[3][2] = b2;
+    h->aI[3][3] = g;
+    h->b[0] = 0;
+    h->b[1] = b1;
+    h->b[2] = b2;
+    h->b[3] = g;
+}
+
+struct Launch {
+    ca_handle *h;
+    hipStream_t st;
+    int id;
+    hipEvent_t a = nullptr, b = nullptr;
+    Launch(ca_handle *h_, hipStream_t st_, int id_) : h(h_), st(st_), id(id_) {
+        if (h->prof) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~Launch() {
+        if (h->prof) {
+            (void)hipEventRecord(b, st);
+            h->recs.push_back({id, a, b});
+        }
+    }
+};
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+ca_status check_launch(ca_handle *h, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, CA_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return CA_OK;
+}
+
+ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
+    if (h->g.nelem == 0 || h->g.ndss == 0) return CA_OK;
+    const DevGeom &g = h->g;
+    {
+        Launch l(h, st, kid);
+        // rho and rho e_tot planes
+        k_dss_scalar<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, Y, 2, g.nlev, 3 * g.nC);
+        k_dss_vector<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, Y + g.nC, Y + 2 * g.nC, g.nlev);
+        k_dss_scalar<<<dim3(cdiv(g.ndss, 256), g.nlev + 1), 256, 0, st>>>(g, Y + 4 * g.nC, 1, g.nlev + 1, 0);
+    }
+    return check_launch(h, "dss");
+}
+
+// remaining_tendency!: Laplacians, their DSS, the fused explicit kernel
+ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const DevNum &n = h->n;
+    if (n.sphere && n.hyper) {
+        {
+            Launch l(h, st, KID_LAPLACIANS);
+            k_laplacians<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, n, Y, h->L);
+        }
+        {
+            Launch l(h, st, KID_DSS_LAPLACIAN);
+            k_dss_vector<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, h->L, h->L + g.nC, g.nlev);
+            k_dss_scalar<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, h->L + 2 * g.nC, 2, g.nlev, g.nC);
+        }
+    }
+    {
+        Launch l(h, st, KID_EXPLICIT);
+        k_explicit<<<cdiv(g.ncol, 256), 256, 0, st>>>(g, n, Y, h->L, Yt);
+    }
+    return check_launch(h, "explicit tendency");
+}
+
+ca_status ensure_bands(ca_handle *h) {
+    if (h->bands) return CA_OK;
+    const DevGeom &g = h->g;
+    ca_status s = dalloc(h, &h->bands, (size_t)(4 * g.nC + 11 * g.nF));
+    if (s) return s;
+    float *p = h->bands;
+    Bands &B = h->B;
+    B.Aru0 = p; p += g.nC;
+    B.Aru1 = p; p += g.nC;
+    B.Aeu0 = p; p += g.nC;
+    B.Aeu1 = p; p += g.nC;
+    float **fp[] = {&B.Aur0, &B.Aur1, &B.Aue0, &B.Aue1, &B.Auh0a, &B.Auh0b, &B.Auh1a, &B.Auh1b, &B.lo, &B.d, &B.up};
+    for (float **x : fp) {
+        *x = p;
+        p += g.nF;
+    }
+    return CA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_config *cfg, ca_handle **out) {
+    if (!geom || !par || !cfg || !out) return CA_ERR_ARG;
+    *out = nullptr;
+    const int nl = geom->nlev, nc = geom->ncol;
+    if (nl < 3 || nc < 1) return CA_ERR_ARG;
+    if (geom->nelem > 0 && nc != 16 * geom->nelem) return CA_ERR_ARG;
+    if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3) return CA_ERR_ARG;
+    if (cfg->vert_diff && !cfg->D0_col) return CA_ERR_ARG;
+    ca_handle *h = new ca_handle();
+    HIPCHK(h, hipGetDevice(&h->device));
+    DevGeom &g = h->g;
+    g.nelem = geom->nelem;
+    g.ncol = nc;
+    g.nlev = nl;
+    g.nC = (int64_t)nl * nc;
+    g.nF = (int64_t)(nl + 1) * nc;
+    h->S = 4 * g.nC + g.nF;
+    auto tofloat = [](const double *p, size_t m) {
+        std::vector<float> v(m);
+        for (size_t i = 0; i < m; ++i) v[i] = (float)p[i];
+        return v;
+    };
+    ca_status s;
+    const float **cp[] = {&g.Jh, &g.gu11, &g.gu12, &g.gu22, &g.gd11, &g.gd12, &g.gd22, &g.f3n, &g.f12a, &g.f12b};
+    std::vector<float> colv(10 * (size_t)nc);
+    for (int i = 0; i < nc; ++i) {
+        colv[0 * (size_t)nc + i] = (float)geom->J_h[i];
+        for (int t = 0; t < 3; ++t) {
+            colv[(1 + t) * (size_t)nc + i] = (float)geom->gup[(size_t)t * nc + i];
+            colv[(4 + t) * (size_t)nc + i] = (float)geom->gdn[(size_t)t * nc + i];
+        }
+        colv[7 * (size_t)nc + i] = (float)geom->f3_num[i];
+        colv[8 * (size_t)nc + i] = (float)geom->f12[i];
+        colv[9 * (size_t)nc + i] = (float)geom->f12[(size_t)nc + i];
+    }
+    float *dcol = nullptr;
+    if ((s = upload(h, &dcol, colv))) return s;
+    for (int t = 0; t < 10; ++t) *cp[t] = dcol + (size_t)t * nc;
+    float *dbasis = nullptr;
+    if ((s = upload(h, &dbasis, tofloat(geom->basis, 12 * (size_t)nc)))) return s;
+    g.basis = dbasis;
+    // vertical tables (computed in double, stored in float)
+    std::vector<double> zf(geom->z_face, geom->z_face + nl + 1);
+    std::vector<float> lc(5 * (size_t)nl), lf(5 * (size_t)(nl + 1));
+    const double R = geom->radius;
+    const double zmax = zf[nl];
+    std::vector<double> zcd(nl);
+    for (int k = 0; k < nl; ++k) zcd[k] = 0.5 * (zf[k] + zf[k + 1]);
+    auto zeta = [&](double z) {
+        if (!(z > cfg->zd_rayleigh)) return 0.0;
+        double sn = std::sin(M_PI * (z - cfg->zd_rayleigh) / (zmax - cfg->zd_rayleigh) / 2);
+        return sn * sn;
+    };
+    for (int k = 0; k < nl; ++k) {
+        lc[0 * (size_t)nl + k] = (float)zcd[k];
+        lc[1 * (size_t)nl + k] = (float)(zf[k + 1] - zf[k]);
+        lc[2 * (size_t)nl + k] = (float)(R > 0 ? (R + zcd[k]) / R : 1.0);
+        lc[3 * (size_t)nl + k] = (float)zeta(zcd[k]);
+        lc[4 * (size_t)nl + k] = (float)(par->grav * zcd[k]);
+    }
+    const int nf = nl + 1;
+    for (int f = 0; f <= nl; ++f) {
+        double Jvf = f == 0 ? zf[1] - zf[0] : (f == nl ? zf[nl] - zf[nl - 1] : zcd[f] - zcd[f - 1]);
+        lf[0 * (size_t)nf + f] = (float)zf[f];
+        lf[1 * (size_t)nf + f] = (float)Jvf;
+        lf[2 * (size_t)nf + f] = (float)(R > 0 ? (R + zf[f]) / R : 1.0);
+        lf[3 * (size_t)nf + f] = (float)(f >= 1 && f < nl ? par->grav * (zcd[f] - zcd[f - 1]) : 0.0);
+        lf[4 * (size_t)nf + f] = (float)zeta(zf[f]);
+    }
+    float *dlc = nullptr, *dlf = nullptr;
+    if ((s = upload(h, &dlc, lc))) return s;
+    if ((s = upload(h, &dlf, lf))) return s;
+    g.zc = dlc;
+    g.Jvc = dlc + nl;
+    g.dzc = dlc + nl;
+    g.sc = dlc + 2 * nl;
+    g.zeta_c = dlc + 3 * nl;
+    g.Phic = dlc + 4 * nl;
+    g.zf = dlf;
+    g.Jvf = dlf + nf;
+    g.sf = dlf + 2 * nf;
+    g.dPhif = dlf + 3 * nf;
+    g.zeta_f = dlf + 4 * nf;
+    g.aw_col = nullptr;
+    g.D0_col = nullptr;
+    if (cfg->alpha_w_col) {
+        float *p = nullptr;
+        if ((s = upload(h, &p, tofloat(cfg->alpha_w_col, nc)))) return s;
+        g.aw_col = p;
+    }
+    if (cfg->vert_diff) {
+        float *p = nullptr;
+        if ((s = upload(h, &p, tofloat(cfg->D0_col, nc)))) return s;
+        g.D0_col = p;
+    }
+    g.ndss = geom->nelem > 0 ? geom->n_dss : 0;
+    if (g.ndss > 0) {
+        int m = geom->dss_ptr[g.ndss];
+        std::vector<int> ptr(geom->dss_ptr, geom->dss_ptr + g.ndss + 1), cols(geom->dss_col, geom->dss_col + m);
+        for (int c : cols)
+            if (c < 0 || c >= nc) {
+                delete h;
+                return CA_ERR_ARG;
+            }
+        int *dp = nullptr, *dcl = nullptr;
+        float *dw = nullptr;
+        if ((s = upload(h, &dp, ptr))) return s;
+        if ((s = upload(h, &dcl, cols))) return s;
+        if ((s = upload(h, &dw, tofloat(geom->dss_w, m)))) return s;
+        g.dptr = dp;
+        g.dcol = dcl;
+        g.dw = dw;
+    }
+    float W[16], Dm[4][4];
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) W[4 * j + i] = (float)(geom->gll_w[i] * geom->gll_w[j]);
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) Dm[i][k] = (float)geom->gll_D[4 * i + k];
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_W), W, sizeof(W)));
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_D), Dm, sizeof(Dm)));
+    DevNum &n = h->n;
+    n.R_d = (float)par->R_d;
+    n.cp_d = (float)par->cp_d;
+    n.cv_d = (float)par->cv_d;
+    n.T_0 = (float)par->T_0;
+    n.p0inv = (float)(1.0 / par->p_ref_theta);
+    n.kex = (float)(par->R_d / par->cp_d);
+    n.kap = (float)(par->R_d / par->cv_d);
+    n.grav = (float)par->grav;
+    n.Tsr = (float)par->T_surf_ref;
+    n.Tmr = (float)par->T_min_ref;
+    n.Tmin = (float)par->T_min_sgs;
+    n.hyper = cfg->hyperdiff && geom->nelem > 0;
+    n.nu_v = (float)cfg->nu4_vort;
+    n.nu_s = (float)cfg->nu4_scal;
+    n.dd = (float)cfg->div_damp;
+    n.upw = cfg->upwinding;
+    n.rayleigh = cfg->rayleigh;
+    n.alpha_uh = (float)cfg->alpha_uh;
+    n.alpha_w = (float)cfg->alpha_w;
+    n.vdiff = cfg->vert_diff;
+    n.dt = (float)cfg->dt;
+    n.sphere = geom->nelem > 0;
+    // work buffers
+    if ((s = dalloc(h, &h->pre, 4 * (size_t)g.nC))) return s;
+    if ((s = dalloc(h, &h->L, 4 * (size_t)g.nC))) return s;
+    if ((s = dalloc(h, &h->scr, 2 * (size_t)g.nF + 6 * (size_t)g.nC))) return s;
+    for (int i = 0; i < 4; ++i)
+        if ((s = dalloc(h, &h->FE[i], (size_t)h->S))) return s;
+    for (int i = 1; i < 4; ++i)
+        if ((s = dalloc(h, &h->FI[i], 2 * (size_t)g.nC + (size_t)g.nF))) return s;
+    if ((s = dalloc(h, &h->U, (size_t)h->S))) return s;
+    if ((s = dalloc(h, &h->flag, 1))) return s;
+    HIPCHK(h, hipMemset(h->L, 0, 4 * (size_t)g.nC * sizeof(float)));
+    ars343(h);
+    *out = h;
+    return CA_OK;
+}
+
+ca_status ca_destroy(ca_handle *h) {
+    if (!h) return CA_ERR_ARG;
+    for (auto &r : h->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (void *p : h->allocs) (void)hipFree(p);
+    delete h;
+    return CA_OK;
+}
+
+int64_t ca_state_size(const ca_handle *h) { return h ? h->S : 0; }
+
+const char *ca_last_error(const ca_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+ca_status ca_set_precomputed(ca_handle *h, float *Y, void *stream) {
+    if (!h || !Y) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    {
+        Launch l(h, st, KID_HOOKS);
+        k_precompute<<<dim3(cdiv(g.ncol, 128), g.nlev), 128, 0, st>>>(g, h->n, Y, h->pre, h->pre + g.nC,
+                                                                       h->pre + 2 * g.nC, h->pre + 3 * g.nC);
+    }
+    return check_launch(h, "k_precompute");
+}
+
+ca_status ca_remaining_tendency(ca_handle *h, float *Yt, const float *Y, void *stream) {
+    if (!h || !Yt || !Y) return CA_ERR_ARG;
+    return explicit_tendency(h, Yt, Y, (hipStream_t)stream);
+}
+
+ca_status ca_implicit_tendency(ca_handle *h, float *Yt, const float *Y, void *stream) {
+    if (!h || !Yt || !Y) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    {
+        Launch l(h, st, KID_HOOKS);
+        k_implicit_tendency<<<dim3(cdiv(g.ncol, 128), g.nlev + 1), 128, 0, st>>>(g, h->n, Yt, Y);
+    }
+    return check_launch(h, "k_implicit_tendency");
+}
+
+ca_status ca_update_jacobian(ca_handle *h, const float *Y, double dtgamma, void *stream) {
+    if (!h || !Y) return CA_ERR_ARG;
+    ca_status s = ensure_bands(h);
+    if (s) return s;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    {
+        Launch l(h, st, KID_HOOKS);
+        k_update_jacobian<<<dim3(cdiv(g.ncol, 128), g.nlev + 1), 128, 0, st>>>(g, h->n, Y, (float)dtgamma, h->B);
+    }
+    return check_launch(h, "k_update_jacobian");
+}
+
+ca_status ca_ldiv(ca_handle *h, float *dY, const float *R, void *stream) {
+    if (!h || !dY || !R) return CA_ERR_ARG;
+    if (!h->bands) return fail(h, CA_ERR_ARG, "ca_ldiv called before ca_update_jacobian");
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    {
+        Launch l(h, st, KID_HOOKS);
+        k_ldiv<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, dY, R, h->scr);
+    }
+    return check_launch(h, "k_ldiv");
+}
+
+ca_status ca_correct_implicit_advection(ca_handle *h, float *Yt, const float *Y, void *stream) {
+    if (!h || !Yt || !Y) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    {
+        Launch l(h, st, KID_HOOKS);
+        k_correct<<<dim3(cdiv(g.ncol, 128), g.nlev), 128, 0, st>>>(g, h->n, Yt, Y);
+    }
+    return check_launch(h, "k_correct");
+}
+
+ca_status ca_dss(ca_handle *h, float *Y, void *stream) {
+    if (!h || !Y) return CA_ERR_ARG;
+    return dss_state(h, Y, (hipStream_t)stream, KID_DSS_STATE);
+}
+
+ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream) {
+    if (!h || !Y || nsteps < 0 || !(dt > 0)) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    ca_status s;
+    // the step assumes a continuous, filtered state (the reference's state
+    // after dss! and cache! at the end of every step)
+    if ((s = dss_state(h, Y, st, KID_DSS_STATE))) return s;
+    {
+        FinalSrc fs{};
+        fs.nfe = 0;
+        fs.nfi = 0;
+        fs.dt = 0.0f;
+        Launch l(h, st, KID_FINAL);
+        k_final<<<cdiv(h->S, 256), 256, 0, st>>>(g, Y, fs);
+    }
+    for (int stp = 0; stp < nsteps; ++stp) {
+        // stage 1: explicit tendency at u
+        if ((s = explicit_tendency(h, h->FE[0], Y, st))) return s;
+        if ((s = dss_state(h, h->FE[0], st, KID_DSS_TENDENCY))) return s;
+        for (int i = 1; i < 4; ++i) {
+            StageSrc src{};
+            src.full[0] = Y;
+            src.cf[0] = 1.0f;
+            src.nfull = 1;
+            for (int j = 0; j < i; ++j)
+                if (h->aE[i][j] != 0.0) {
+                    src.full[src.nfull] = h->FE[j];
+                    src.cf[src.nfull] = (float)(dt * h->aE[i][j]);
+                    ++src.nfull;
+                }
+            src.nfi = 0;
+            for (int j = 1; j < i; ++j)
+                if (h->aI[i][j] != 0.0) {
+                    src.fi[src.nfi] = h->FI[j];
+                    src.ci[src.nfi] = (float)(dt * h->aI[i][j]);
+                    ++src.nfi;
+                }
+            float dtg = (float)(dt * h->aI[i][i]);
+            {
+                Launch l(h, st, KID_NEWTON_STAGE);
+                k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i], h->scr);
+            }
+            if ((s = check_launch(h, "k_newton_stage"))) return s;
+            if ((s = explicit_tendency(h, h->FE[i], h->U, st))) return s;
+            if ((s = dss_state(h, h->FE[i], st, KID_DSS_TENDENCY))) return s;
+        }
+        FinalSrc fs{};
+        fs.dt = (float)dt;
+        for (int j = 0; j < 4; ++j)
+            if (h->b[j] != 0.0) {
+                fs.fe[fs.nfe] = h->FE[j];
+                fs.be[fs.nfe] = (float)h->b[j];
+                ++fs.nfe;
+                if (j >= 1) {
+                    fs.fi[fs.nfi] = h->FI[j];
+                    fs.bi[fs.nfi] = (float)h->b[j];
+                    ++fs.nfi;
+                }
+            }
+        {
+            Launch l(h, st, KID_FINAL);
+            k_final<<<cdiv(h->S, 256), 256, 0, st>>>(g, Y, fs);
+        }
+        if ((s = check_launch(h, "k_final"))) return s;
+    }
+    return CA_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------- layout
+__global__ void k_pack(DevGeom g, const float *src, ca_layout lay, int nlev_c, float *dst, int dir) {
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y;
+    if (col >= g.ncol || k >= nlev_c) return;
+    int e, i, j;
+    if (g.nelem > 0) {
+        e = col >> 4;
+        j = (col >> 2) & 3;
+        i = col & 3;
+    } else {
+        e = col;
+        i = j = 0;
+    }
+    int64_t so = lay.offset + k * lay.s_v + i * lay.s_i + j * lay.s_j + (int64_t)e * lay.s_h;
+    int64_t d = (int64_t)k * g.ncol + col;
+    if (dir == 0)
+        dst[d] = src[so];
+    else
+        dst[so] = src[d];
+}
+
+__global__ void k_nonfinite(const float *Y, int64_t S, int *flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S && !isfinite(Y[i])) atomicOr(flag, 1);
+}
+
+extern "C" {
+
+ca_status ca_pack_state(ca_handle *h, const float *src, const ca_layout *lay, float *Y, void *stream) {
+    if (!h || !src || !lay || !Y) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    for (int c = 0; c < 5; ++c) {
+        int nl = c < 4 ? g.nlev : g.nlev + 1;
+        k_pack<<<dim3(cdiv(g.ncol, 128), nl), 128, 0, st>>>(g, src, lay[c], nl, Y + (int64_t)c * g.nC, 0);
+    }
+    return check_launch(h, "k_pack");
+}
+
+ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *lay, float *dst, void *stream) {
+    if (!h || !dst || !lay || !Y) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    for (int c = 0; c < 5; ++c) {
+        int nl = c < 4 ? g.nlev : g.nlev + 1;
+        k_pack<<<dim3(cdiv(g.ncol, 128), nl), 128, 0, st>>>(g, Y + (int64_t)c * g.nC, lay[c], nl, dst, 1);
+    }
+    return check_launch(h, "k_unpack");
+}
+
+ca_status ca_check_finite(ca_handle *h, const float *Y, int32_t *flag, void *stream) {
+    if (!h || !Y || !flag) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(h, hipMemsetAsync(h->flag, 0, sizeof(int), st));
+    k_nonfinite<<<cdiv(h->S, 256), 256, 0, st>>>(Y, h->S, h->flag);
+    ca_status s = check_launch(h, "k_nonfinite");
+    if (s) return s;
+    int v = 0;
+    HIPCHK(h, hipMemcpyAsync(&v, h->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    *flag = v;
+    return v ? fail(h, CA_ERR_NONFINITE, "state holds NaN or Inf") : CA_OK;
+}
+
+ca_status ca_profile_enable(ca_handle *h, int32_t enable) {
+    if (!h) return CA_ERR_ARG;
+    HIPCHK(h, hipDeviceSynchronize());
+    for (auto &r : h->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    h->recs.clear();
+    h->prof = enable != 0;
+    return CA_OK;
+}
+
+int32_t ca_profile_count(void) { return KID_COUNT; }
+
+ca_status ca_profile_get(ca_handle *h, int32_t id, const char **name, int64_t *launches, double *total_ms) {
+    if (!h || id < 0 || id >= KID_COUNT) return CA_ERR_ARG;
+    HIPCHK(h, hipDeviceSynchronize());
+    double tot = 0.0;
+    int64_t cnt = 0;
+    for (auto &r : h->recs)
+        if (r.id == id) {
+            float ms = 0.0f;
+            HIPCHK(h, hipEventElapsedTime(&ms, r.a, r.b));
+            tot += ms;
+            ++cnt;
+        }
+    if (name) *name = kKernelNames[id];
+    if (launches) *launches = cnt;
+    if (total_ms) *total_ms = tot;
+    return CA_OK;
+}
+
+}  // extern "C"

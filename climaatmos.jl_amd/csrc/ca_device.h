@@ -1,0 +1,181 @@
+// ca_device.h -- device-side data structures and pointwise physics of the
+// dry-dycore step (gfx950, float32).
+//
+// Layout: every field is level-major, value(k, col) at [k * ncol + col];
+// col = 16 * element + 4 * j + i.  Geometry is factored: per-column
+// horizontal metric planes and per-level vertical tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CA_NQ 4
+#define CA_NN 16
+
+// GLL Nq = 4: quadrature weight products W[4 j + i] = w_i w_j and the
+// derivative matrix D[i][k] = l_k'(xi_i).  Filled once by ca_create.
+__constant__ float c_W[CA_NN];
+__constant__ float c_D[CA_NQ][CA_NQ];
+
+struct DevGeom {
+    int nelem, ncol, nlev;
+    int64_t nC, nF;  // centre / face plane sizes
+    // per-column planes (ncol each)
+    const float *Jh, *gu11, *gu12, *gu22, *gd11, *gd12, *gd22, *f3n, *f12a, *f12b;
+    const float *basis;  // 12 planes: a^1 (x,y,z), a^2, a_1, a_2
+    // per-level tables
+    const float *zc, *Jvc, *sc, *dzc;    // centres [nlev]; dzc = Jvc
+    const float *zf, *Jvf, *sf, *dPhif;  // faces [nlev + 1]; dPhif[f] = g (zc[f] - zc[f-1])
+    const float *zeta_c, *zeta_f;        // Rayleigh profiles sin^2(...) (0 below zd)
+    const float *Phic;                   // g zc
+    const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
+    // DSS groups
+    int ndss;
+    const int *dptr, *dcol;
+    const float *dw;
+};
+
+struct DevNum {
+    float R_d, cp_d, cv_d, T_0, p0inv, kex, kap, grav, Tsr, Tmr, Tmin;
+    float nu_v, nu_s, dd;
+    float alpha_uh, alpha_w, dt;
+    int hyper, upw, rayleigh, vdiff, sphere;
+};
+
+// ------------------------------------------------------------- thermodynamics
+// Dry air (Thermodynamics 1.3.0 as used at precomputed_quantities.jl:786-815):
+// e_int = cv_d (T - T_0) - R_d T_0, p = rho R_d T, h_tot = e_tot + R_d T,
+// Pi = (p / p_ref)^(R_d / cp_d); T is floored at T_min_sgs (:786-796).
+__device__ __forceinline__ float ca_temperature(const DevNum &n, float e_int) {
+    float T = n.T_0 + (e_int + n.R_d * n.T_0) / n.cv_d;
+    return fmaxf(T, n.Tmin);
+}
+__device__ __forceinline__ float ca_exner(const DevNum &n, float p) { return powf(p * n.p0inv, n.kex); }
+// reference temperature T_r(Pi) = T_min + (T_sfc - T_min) Pi^7 (refstate_thermodynamics.jl:80-88)
+__device__ __forceinline__ float ca_pow7(float x) {
+    float x2 = x * x;
+    float x4 = x2 * x2;
+    return x4 * x2 * x;
+}
+__device__ __forceinline__ float ca_Tref(const DevNum &n, float Pi) { return n.Tmr + (n.Tsr - n.Tmr) * ca_pow7(Pi); }
+// Phi_r(p_b) - Phi_r(p_a) in a cancellation-free form:
+//   -cp_d [T_min ln(Pi_b / Pi_a) + (T_sfc - T_min) / 7 (Pi_b^7 - Pi_a^7)]
+// with ln(Pi_b / Pi_a) = kex ln(p_b / p_a) and Pi_b^7 - Pi_a^7 = Pi_a^7 expm1(7 ln(Pi_b / Pi_a)).
+__device__ __forceinline__ float ca_dPhiref(const DevNum &n, float pa, float Pia, float pb) {
+    float lr = n.kex * log1pf((pb - pa) / pa);
+    return -n.cp_d * (n.Tmr * lr + (n.Tsr - n.Tmr) * (1.0f / 7.0f) * ca_pow7(Pia) * expm1f(7.0f * lr));
+}
+// Pi_b - Pi_a = Pi_a expm1(kex ln(p_b / p_a))
+__device__ __forceinline__ float ca_dExner(const DevNum &n, float pa, float Pia, float pb) {
+    return Pia * expm1f(n.kex * log1pf((pb - pa) / pa));
+}
+
+// horizontal metric at scale s (no topography): contravariant u^a = g^{ab} u_b / s^2
+__device__ __forceinline__ void ca_contra(const DevGeom &g, int col, float is2, float a1, float a2, float &o1,
+                                          float &o2) {
+    float g11 = g.gu11[col], g12 = g.gu12[col], g22 = g.gu22[col];
+    o1 = is2 * (g11 * a1 + g12 * a2);
+    o2 = is2 * (g12 * a1 + g22 * a2);
+}
+__device__ __forceinline__ void ca_cov(const DevGeom &g, int col, float s2, float a1, float a2, float &o1,
+                                       float &o2) {
+    float g11 = g.gd11[col], g12 = g.gd12[col], g22 = g.gd22[col];
+    o1 = s2 * (g11 * a1 + g12 * a2);
+    o2 = s2 * (g12 * a1 + g22 * a2);
+}
+
+// ------------------------------------------------- element-local derivatives
+// The 16 nodes of an element occupy 16 consecutive lanes of a wavefront
+// (lane = 16 e + 4 j + i).  Values of neighbouring nodes are exchanged with
+// cross-lane permutes; D stays in constant memory.
+struct Nod {
+    int base, i, j;  // lane of node (0, 0) of the element; node indices
+};
+__device__ __forceinline__ Nod ca_node(int lane) {
+    Nod q;
+    q.base = lane & ~15;
+    q.i = lane & 3;
+    q.j = (lane >> 2) & 3;
+    return q;
+}
+__device__ __forceinline__ float ca_sh(float v, int src) { return __shfl(v, src, 64); }
+// sum_m D[i][m] f(m, j)
+__device__ __forceinline__ float ca_d1(const Nod &q, float f) {
+    float r = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r = fmaf(c_D[q.i][m], ca_sh(f, q.base + 4 * q.j + m), r);
+    return r;
+}
+// sum_m D[j][m] f(i, m)
+__device__ __forceinline__ float ca_d2(const Nod &q, float f) {
+    float r = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r = fmaf(c_D[q.j][m], ca_sh(f, q.base + 4 * m + q.i), r);
+    return r;
+}
+// sum_m D[m][i] g(m, j)   (transpose: weak forms)
+__device__ __forceinline__ float ca_d1t(const Nod &q, float f) {
+    float r = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r = fmaf(c_D[m][q.i], ca_sh(f, q.base + 4 * q.j + m), r);
+    return r;
+}
+__device__ __forceinline__ float ca_d2t(const Nod &q, float f) {
+    float r = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r = fmaf(c_D[m][q.j], ca_sh(f, q.base + 4 * m + q.i), r);
+    return r;
+}
+
+// Spectral-element operators at one level (docs/src/discretization.md:72-135).
+// J = 3-D Jacobian at the node, W = quadrature weight product.
+// strong gradient -> C12
+__device__ __forceinline__ void op_grad(const Nod &q, float f, float &o1, float &o2) {
+    o1 = ca_d1(q, f);
+    o2 = ca_d2(q, f);
+}
+// strong divergence of contravariant (v1, v2)
+__device__ __forceinline__ float op_div(const Nod &q, float J, float v1, float v2) {
+    return (ca_d1(q, J * v1) + ca_d2(q, J * v2)) / J;
+}
+// weak divergence: -(D^T (W J v1) + D^T (W J v2)) / (W J)
+__device__ __forceinline__ float op_wdiv(const Nod &q, float W, float J, float v1, float v2) {
+    float WJ = W * J;
+    return -(ca_d1t(q, WJ * v1) + ca_d2t(q, WJ * v2)) / WJ;
+}
+// weak gradient -> C12: -(D^T (W f)) / W
+__device__ __forceinline__ void op_wgrad(const Nod &q, float W, float f, float &o1, float &o2) {
+    float Wf = W * f;
+    o1 = -ca_d1t(q, Wf) / W;
+    o2 = -ca_d2t(q, Wf) / W;
+}
+// strong curl of C12 (u1, u2) -> CT3
+__device__ __forceinline__ float op_curl3(const Nod &q, float J, float u1, float u2) {
+    return (ca_d1(q, u2) - ca_d2(q, u1)) / J;
+}
+// weak curl of C12 -> CT3: (-D^T (W u2) + D^T (W u1)) / (W J)
+__device__ __forceinline__ float op_wcurl3(const Nod &q, float W, float J, float u1, float u2) {
+    return (-ca_d1t(q, W * u2) + ca_d2t(q, W * u1)) / (W * J);
+}
+// strong curl of C3 u3 -> CT12
+__device__ __forceinline__ void op_curl12(const Nod &q, float J, float u3, float &o1, float &o2) {
+    o1 = ca_d2(q, u3) / J;
+    o2 = -ca_d1(q, u3) / J;
+}
+// weak curl of C3 -> CT12
+__device__ __forceinline__ void op_wcurl12(const Nod &q, float W, float J, float u3, float &o1, float &o2) {
+    float Wu = W * u3;
+    float WJ = W * J;
+    o1 = -ca_d2t(q, Wu) / WJ;
+    o2 = ca_d1t(q, Wu) / WJ;
+}
+
+// van Leer (Lin et al. 1994) slope with the monotone local-extrema
+// constraint, on the upwind cell a1 with neighbours a0, a2.
+__device__ __forceinline__ float ca_vl_slope(float a0, float a1, float a2) {
+    float d = 0.5f * ((a1 - a0) + (a2 - a1));
+    float mn = fminf(fminf(a0, a1), a2), mx = fmaxf(fmaxf(a0, a1), a2);
+    float lim = fminf(fabsf(d), fminf(2.0f * (a1 - mn), 2.0f * (mx - a1)));
+    return d > 0.0f ? lim : (d < 0.0f ? -lim : 0.0f);
+}
+
+#define CA_CHECK_LAUNCH() (void)0

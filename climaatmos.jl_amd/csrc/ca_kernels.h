@@ -1,0 +1,55 @@
+// ca_kernels.h -- kernel declarations and launch-argument structures.
+#pragma once
+#include "ca_device.h"
+
+// centre quantities of one column level (recomputed in registers, never stored)
+struct CV {
+    float rho, u1, u2, rhoe;
+    float ct1, ct2;  // contravariant horizontal velocity
+    float K, T, h, p, Pi, J;
+};
+
+// Jacobian bands (update_jacobian! storage for the hook-level ldiv)
+struct Bands {
+    float *Aru0, *Aru1, *Aeu0, *Aeu1;                 // centre rows
+    float *Aur0, *Aur1, *Aue0, *Aue1;                 // face rows: centres f-1, f
+    float *Auh0a, *Auh0b, *Auh1a, *Auh1b;
+    float *lo, *d, *up;
+};
+
+#define CA_MAX_SRC 8
+// stage-state assembly: U~ = sum cf[t] * full[t] + sum ci[t] * fi[t]
+// (fi sources hold the rho, rho e_tot and u_3 planes of an implicit tendency)
+struct StageSrc {
+    const float *full[CA_MAX_SRC];
+    float cf[CA_MAX_SRC];
+    const float *fi[CA_MAX_SRC];
+    float ci[CA_MAX_SRC];
+    int nfull, nfi;
+};
+
+struct FinalSrc {
+    const float *fe[CA_MAX_SRC];
+    float be[CA_MAX_SRC];
+    const float *fi[CA_MAX_SRC];
+    float bi[CA_MAX_SRC];
+    int nfe, nfi;
+    float dt;
+};
+
+// column kernels (ca_column.hip)
+__global__ void k_precompute(DevGeom g, DevNum n, float *Y, float *K, float *T, float *p, float *h);
+__global__ void k_implicit_tendency(DevGeom g, DevNum n, float *Yt, const float *Y);
+__global__ void k_update_jacobian(DevGeom g, DevNum n, const float *Y, float dtg, Bands B);
+__global__ void k_ldiv(DevGeom g, Bands B, float *dY, const float *R, float *scr);
+__global__ void k_correct(DevGeom g, DevNum n, float *Yt, const float *Y);
+__global__ void k_newton_stage(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI, float *scr);
+__global__ void k_final(DevGeom g, float *u, FinalSrc s);
+
+// explicit tendency (ca_spectral.hip)
+__global__ void k_laplacians(DevGeom g, DevNum n, const float *Y, float *L);
+__global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt);
+
+// DSS (ca_dss.hip): weighted average over shared nodes, level range of planes
+__global__ void k_dss_scalar(DevGeom g, float *f, int nplanes, int nlev_plane, int64_t plane_stride);
+__global__ void k_dss_vector(DevGeom g, float *u1, float *u2, int nlev_plane);

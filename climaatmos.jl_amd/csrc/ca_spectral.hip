@@ -1,0 +1,294 @@
+// ca_spectral.hip -- the explicit ("remaining") tendency of the dry dycore:
+// horizontal spectral-element dynamics, fourth-order hyperdiffusion,
+// vector-invariant vertical advection with deep-atmosphere Coriolis, and the
+// dry additional tendencies (Rayleigh u_h sponge, K-theory vertical diffusion).
+//
+// Restated from (reference paths):
+//   src/prognostic_equations/remaining_tendency.jl:48-58
+//   src/prognostic_equations/advection.jl:36-91, 205-290
+//   src/prognostic_equations/hyperdiffusion.jl:116-147, 247-307
+//   src/prognostic_equations/vertical_diffusion_boundary_layer.jl:63-103
+//   src/parameterized_tendencies/sponge/rayleigh_sponge.jl:39-44
+//
+// Mapping: one lane per horizontal node, the 16 nodes of an element in 16
+// consecutive lanes; horizontal derivatives are cross-lane permutes, vertical
+// stencils are a register window carried up the column.
+#include "ca_kernels.h"
+
+__device__ __forceinline__ float u3f_at(const DevGeom &g, const float *u3, int f, int col) {
+    return (f <= 0 || f >= g.nlev) ? 0.0f : u3[(int64_t)f * g.ncol + col];
+}
+
+// ----------------------------------------------------------- prep kernel
+// prep_hyperdiffusion_tendency! (hyperdiffusion.jl:141-147, dry):
+//   L_u = C123(wgrad(div u)) - C123(wcurl(C123(curl u)))   (3 planes)
+//   L_s = wdiv(grad(s_d - s_d,r))                          (1 plane)
+// Thread per (node, centre level).  The s_d - s_d,r field is taken relative
+// to its value at the element's first node (a constant inside the element,
+// annihilated by the gradient), which removes the large geopotential part.
+__global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const float *Y, float *L) {
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y;
+    if (col >= g.ncol) return;
+    const Nod q = ca_node(threadIdx.x & 63);
+    const int64_t nC = g.nC;
+    const int64_t idx = (int64_t)k * g.ncol + col;
+    const float *u3 = Y + 4 * nC;
+    float rho = Y[idx], u1 = Y[nC + idx], u2 = Y[2 * nC + idx], rhoe = Y[3 * nC + idx];
+    float u3a = u3f_at(g, u3, k, col), u3b = u3f_at(g, u3, k + 1, col);
+    float s = g.sc[k], s2 = s * s, is2 = 1.0f / s2, Jv = g.Jvc[k];
+    float J = g.Jh[col] * s2 * Jv;
+    float W = c_W[q.j * 4 + q.i];
+    float ct1, ct2;
+    ca_contra(g, col, is2, u1, u2, ct1, ct2);
+    float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]), gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
+    float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
+    float e_tot = rhoe / rho;
+    float T = ca_temperature(n, e_tot - K - g.Phic[k]);
+    float p = rho * n.R_d * T;
+    float Pi = ca_exner(n, p);
+    float cu3 = 0.5f * (u3a + u3b);
+    // vector Laplacian of c_u
+    float dv = op_div(q, J, ct1, ct2);
+    float gd1, gd2;
+    op_wgrad(q, W, dv, gd1, gd2);
+    float cr3 = op_curl3(q, J, u1, u2);
+    float cr1, cr2;
+    op_curl12(q, J, cu3, cr1, cr2);
+    float cc1, cc2;
+    ca_cov(g, col, s2, cr1, cr2, cc1, cc2);
+    float cc3 = Jv * Jv * cr3;
+    float w3 = op_wcurl3(q, W, J, cc1, cc2);
+    float w1, w2;
+    op_wcurl12(q, W, J, cc3, w1, w2);
+    float wc1, wc2;
+    ca_cov(g, col, s2, w1, w2, wc1, wc2);
+    float wc3 = Jv * Jv * w3;
+    L[idx] = gd1 - wc1;
+    L[nC + idx] = gd2 - wc2;
+    L[2 * nC + idx] = 0.0f - wc3;
+    // scalar Laplacian of the dry static energy deviation
+    float p0 = ca_sh(p, q.base), Pi0 = ca_sh(Pi, q.base);
+    float psi = n.cp_d * (T - ca_Tref(n, Pi)) - ca_dPhiref(n, p0, Pi0, p);
+    float q1, q2;
+    op_grad(q, psi, q1, q2);
+    float v1, v2;
+    ca_contra(g, col, is2, q1, q2, v1, v2);
+    L[3 * nC + idx] = op_wdiv(q, W, J, v1, v2);
+}
+
+// --------------------------------------------------------- explicit kernel
+struct ColState {
+    float rhoJ, K, ct1, ct2, u1, u2, rho;
+    float l4;                  // C3 of nabla^4 u (hyperdiffusion)
+    float dr, d1, d2, de;      // partial centre tendencies
+    float sd;                  // dry static energy (vertical diffusion)
+    float uc0, uc1, uc2;       // velocity in the diffusion basis
+};
+
+__global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt) {
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= g.ncol) return;
+    const Nod q = ca_node(threadIdx.x & 63);
+    const int nl = g.nlev;
+    const int64_t nC = g.nC, nc = g.ncol;
+    const float *u3 = Y + 4 * nC;
+    const bool sph = n.sphere != 0;
+    const float W = c_W[q.j * 4 + q.i];
+    const float Jh = g.Jh[col];
+    const float f3n = g.f3n[col], f12a = g.f12a[col], f12b = g.f12b[col];
+    float Kh = 0.0f;
+    if (n.vdiff) Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
+    ColState prev, cur;
+    prev.rhoJ = prev.K = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
+    prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = 0.0f;
+    float Gp1 = 0.0f, Gp2 = 0.0f;      // G at the face below the current centre
+    float Fs_prev = 0.0f, Fu0_prev = 0.0f, Fu1_prev = 0.0f, Fu2_prev = 0.0f;  // diffusive fluxes
+    float u3a = 0.0f;                  // u_3 at face k
+    for (int k = 0; k <= nl; ++k) {
+        float u3b = (k < nl) ? u3f_at(g, u3, k + 1, col) : 0.0f;
+        if (k < nl) {
+            const int64_t idx = (int64_t)k * nc + col;
+            float rho = Y[idx], u1 = Y[nC + idx], u2 = Y[2 * nC + idx], rhoe = Y[3 * nC + idx];
+            float s = g.sc[k], s2 = s * s, is2 = 1.0f / s2, Jv = g.Jvc[k];
+            float J = Jh * s2 * Jv;
+            float ct1, ct2;
+            ca_contra(g, col, is2, u1, u2, ct1, ct2);
+            float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]), gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
+            float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
+            float e_tot = rhoe / rho;
+            float T = ca_temperature(n, e_tot - K - g.Phic[k]);
+            float h = e_tot + n.R_d * T;
+            float p = rho * n.R_d * T;
+            float Pi = ca_exner(n, p);
+            float dr = 0.0f, d1 = 0.0f, d2 = 0.0f, de = 0.0f, l4 = 0.0f;
+            if (sph) {
+                // ---- horizontal_dynamics_tendency! (advection.jl:36-91)
+                float m1 = rho * ct1, m2 = rho * ct2;
+                float wd = op_wdiv(q, W, J, m1, m2);
+                dr -= 0.5f * wd + 0.5f * wd;  // split_div(rho u, 1)
+                // split_div(rho u, h) = 1/2 wdiv(rho u h) + 1/2 (h wdiv(rho u) + rho u . grad h),
+                // with h taken relative to the element's first node (exact: linear in constants)
+                float h0 = ca_sh(h, q.base);
+                float hh = h - h0;
+                float wdh = op_wdiv(q, W, J, m1 * hh, m2 * hh);
+                float gh1, gh2;
+                op_grad(q, hh, gh1, gh2);
+                de -= 0.5f * wdh + 0.5f * (hh * wd + (m1 * gh1 + m2 * gh2)) + h0 * wd;
+                // u_h -= C12(grad(K + Phi - Phi_r) + cp_d (th' grad Pi + grad(th' Pi) - Pi grad th') / 2)
+                float p0 = ca_sh(p, q.base), Pi0 = ca_sh(Pi, q.base);
+                float A = K - ca_dPhiref(n, p0, Pi0, p);
+                float dPi = ca_dExner(n, p0, Pi0, p);
+                float thPi = T - ca_Tref(n, Pi);
+                float th = thPi / Pi;
+                float g1, g2, a1, a2, b1, b2, c1, c2;
+                op_grad(q, A, g1, g2);
+                op_grad(q, dPi, a1, a2);
+                op_grad(q, thPi, b1, b2);
+                op_grad(q, th, c1, c2);
+                d1 -= g1 + n.cp_d * (th * a1 + b1 - Pi * c1) * 0.5f;
+                d2 -= g2 + n.cp_d * (th * a2 + b2 - Pi * c2) * 0.5f;
+                // ---- apply_hyperdiffusion_tendency! (hyperdiffusion.jl:273-307)
+                if (n.hyper) {
+                    float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
+                    float v1, v2;
+                    ca_contra(g, col, is2, X1, X2, v1, v2);
+                    float dv = op_div(q, J, v1, v2);
+                    float gd1, gd2;
+                    op_wgrad(q, W, dv, gd1, gd2);
+                    float cr3 = op_curl3(q, J, X1, X2);
+                    float cr1, cr2;
+                    op_curl12(q, J, X3, cr1, cr2);
+                    float cc1, cc2;
+                    ca_cov(g, col, s2, cr1, cr2, cc1, cc2);
+                    float cc3 = Jv * Jv * cr3;
+                    float w3 = op_wcurl3(q, W, J, cc1, cc2);
+                    float w1, w2;
+                    op_wcurl12(q, W, J, cc3, w1, w2);
+                    float wc1, wc2;
+                    ca_cov(g, col, s2, w1, w2, wc1, wc2);
+                    float wc3 = Jv * Jv * w3;
+                    d1 -= n.nu_v * (n.dd * gd1 - wc1);
+                    d2 -= n.nu_v * (n.dd * gd2 - wc2);
+                    l4 = n.dd * 0.0f - wc3;
+                    float s1, s2g;
+                    op_grad(q, Ls, s1, s2g);
+                    float r1, r2;
+                    ca_contra(g, col, is2, rho * s1, rho * s2g, r1, r2);
+                    de -= n.nu_s * op_wdiv(q, W, J, r1, r2);
+                }
+            }
+            // ---- explicit_vertical_advection_tendency! centre part (advection.jl:273-277)
+            float om3 = sph ? op_wcurl3(q, W, J, u1, u2) : 0.0f;
+            float a3 = f3n / Jv + om3;
+            d1 -= J * (0.0f - a3 * ct2);
+            d2 -= J * (a3 * ct1 - 0.0f);
+            // ---- Rayleigh sponge on u_h (rayleigh_sponge.jl:39-44)
+            if (n.rayleigh && n.alpha_uh != 0.0f) {
+                float beta = n.alpha_uh * g.zeta_c[k];
+                d1 += -beta * u1;
+                d2 += -beta * u2;
+            }
+            cur.rhoJ = rho * J;
+            cur.K = K;
+            cur.ct1 = ct1;
+            cur.ct2 = ct2;
+            cur.u1 = u1;
+            cur.u2 = u2;
+            cur.rho = rho;
+            cur.l4 = l4;
+            cur.dr = dr;
+            cur.d1 = d1;
+            cur.d2 = d2;
+            cur.de = de;
+            if (n.vdiff) {
+                cur.sd = n.cp_d * (T - n.T_0) + g.Phic[k];
+                if (sph) {
+                    const float *B = g.basis;
+                    float is = 1.0f / s;
+                    cur.uc0 = (u1 * B[0 * nc + col] + u2 * B[3 * nc + col]) * is;
+                    cur.uc1 = (u1 * B[1 * nc + col] + u2 * B[4 * nc + col]) * is;
+                    cur.uc2 = (u1 * B[2 * nc + col] + u2 * B[5 * nc + col]) * is;
+                } else {
+                    cur.uc0 = u1;
+                    cur.uc1 = u2;
+                    cur.uc2 = 0.0f;
+                }
+            }
+        }
+        // ---- face k (advection.jl:278): u_3 tendency and the face term G
+        float G1 = 0.0f, G2 = 0.0f, du3 = 0.0f;
+        float Fs = 0.0f, Fu0 = 0.0f, Fu1 = 0.0f, Fu2 = 0.0f;
+        {
+            // the horizontal weak curl of u_3 at face k needs every lane of the element
+            float x = 0.0f, y = 0.0f;
+            const bool interior = (k >= 1 && k <= nl - 1);
+            float sf = g.sf[k], Jf = Jh * sf * sf * g.Jvf[k];
+            if (sph && interior) op_wcurl12(q, W, Jf, u3a, x, y);
+            if (interior) {
+                float fu3 = u3a / (g.Jvf[k] * g.Jvf[k]);
+                float wa = -(cur.u2 - prev.u2) / Jf + x;
+                float wb = (cur.u1 - prev.u1) / Jf + y;
+                float A1 = f12a / sf + wa, A2 = f12b / sf + wb;
+                float rJf = 0.5f * (prev.rhoJ + cur.rhoJ);
+                float b3 = rJf * fu3;
+                G1 = Jf * (A2 * b3);
+                G2 = Jf * (0.0f - A1 * b3);
+                float i1 = 0.5f * (prev.ct1 + cur.ct1), i2 = 0.5f * (prev.ct2 + cur.ct2);
+                du3 = -(Jf * (A1 * i2 - A2 * i1) + (cur.K - prev.K));
+                if (sph && n.hyper) {
+                    float wA = prev.rhoJ, wB = cur.rhoJ;
+                    du3 -= n.nu_v * ((0.5f * (wA * prev.l4 + wB * cur.l4)) / (0.5f * (wA + wB)));
+                }
+                if (n.vdiff) {
+                    // vertical_diffusion_boundary_layer.jl:90-103 with K_h = D0 (H = Inf)
+                    float rK = (0.5f * (prev.rho + cur.rho)) / (0.5f * (1.0f / Kh + 1.0f / Kh));
+                    float iJv = 1.0f / g.Jvf[k];
+                    Fs = Jf * (iJv * iJv * (-(rK * (cur.sd - prev.sd))));
+                    float fac = Jf * iJv * (-2.0f * rK) * 0.5f;
+                    Fu0 = fac * (cur.uc0 - prev.uc0) * iJv;
+                    Fu1 = fac * (cur.uc1 - prev.uc1) * iJv;
+                    Fu2 = fac * (cur.uc2 - prev.uc2) * iJv;
+                }
+            }
+            Yt[4 * nC + (int64_t)k * nc + col] = du3;
+        }
+        // ---- centre k-1 is complete
+        if (k >= 1) {
+            const int kk = k - 1;
+            const int64_t idx = (int64_t)kk * nc + col;
+            float d1 = prev.d1 - (0.5f * (Gp1 + G1)) / prev.rhoJ;
+            float d2 = prev.d2 - (0.5f * (Gp2 + G2)) / prev.rhoJ;
+            float de = prev.de;
+            if (n.vdiff) {
+                float s = g.sc[kk];
+                float Jc = Jh * s * s * g.Jvc[kk];
+                de -= (Fs - Fs_prev) / Jc;
+                float e0 = (Fu0 - Fu0_prev) / Jc, e1 = (Fu1 - Fu1_prev) / Jc, e2 = (Fu2 - Fu2_prev) / Jc;
+                float c1, c2;
+                if (sph) {
+                    const float *B = g.basis;
+                    c1 = s * (e0 * B[6 * nc + col] + e1 * B[7 * nc + col] + e2 * B[8 * nc + col]);
+                    c2 = s * (e0 * B[9 * nc + col] + e1 * B[10 * nc + col] + e2 * B[11 * nc + col]);
+                } else {
+                    c1 = e0;
+                    c2 = e1;
+                }
+                d1 -= c1 / prev.rho;
+                d2 -= c2 / prev.rho;
+            }
+            Yt[idx] = prev.dr;
+            Yt[nC + idx] = d1;
+            Yt[2 * nC + idx] = d2;
+            Yt[3 * nC + idx] = de;
+        }
+        Gp1 = G1;
+        Gp2 = G2;
+        Fs_prev = Fs;
+        Fu0_prev = Fu0;
+        Fu1_prev = Fu1;
+        Fu2_prev = Fu2;
+        prev = cur;
+        u3a = u3b;
+    }
+}

@@ -1,0 +1,144 @@
+/*
+ * cadycore.h -- C ABI of the MI355X-native ClimaAtmos dry-dycore step.
+ *
+ * The entry points replace, one for one, the hooks ClimaTimeSteppers calls on
+ * the ClimaAtmos ODE function built in src/simulation/integrator.jl:193-225:
+ *
+ *   ca_remaining_tendency          remaining_tendency!(Yt, Yt_lim, Y, p, t)
+ *                                  src/prognostic_equations/remaining_tendency.jl:48
+ *   ca_implicit_tendency           implicit_tendency!(Yt, Y, p, t)
+ *                                  src/prognostic_equations/implicit/implicit_tendency.jl:36
+ *   ca_update_jacobian             update_jacobian!(jac, Y, p, dtgamma, t)   (Wfact)
+ *                                  src/prognostic_equations/implicit/jacobian.jl:74-75
+ *   ca_ldiv                        LinearAlgebra.ldiv!(dY, jac, R)
+ *                                  src/prognostic_equations/implicit/jacobian.jl:78-82
+ *   ca_correct_implicit_advection  correct_implicit_advection_tendency!(Yt, Y, p, t) (T_post_imp!)
+ *                                  src/prognostic_equations/implicit/implicit_tendency.jl:322
+ *   ca_set_precomputed             set_implicit_precomputed_quantities!(Y, p, t)  (cache_imp!)
+ *                                  src/cache/precomputed_quantities.jl:698
+ *   ca_dss                         dss!(Y, p, t)  src/prognostic_equations/constrain_state.jl:59
+ *   ca_step                        one IMEXAlgorithm(ARS343, NewtonsMethod(max_iters = 1)) step,
+ *                                  as driven by CTS.step! from solve_atmos!
+ *                                  (src/simulation/solve.jl:119-158)
+ *
+ * All state pointers are DEVICE pointers to float32 data in the native
+ * layout: [rho | u_1 | u_2 | rho e_tot] centre planes (nlev x ncol each,
+ * level-major) followed by the u_3 face plane ((nlev + 1) x ncol), i.e. the
+ * reference's Y.c = (rho, u_h::C12, rho e_tot) and Y.f = (u_3::C3).
+ * ca_pack_state / ca_unpack_state convert from / to any strided layout
+ * (ClimaCore's VIJFH / VIJHF parent arrays included).
+ *
+ * Calls are stream-ordered on the hipStream_t passed as `stream` (NULL = the
+ * default stream).  A handle is not thread-safe: the caller serialises calls
+ * on one handle, as ClimaTimeSteppers does.  Nothing throws across the ABI;
+ * every call returns a ca_status and ca_last_error() describes the last
+ * failure.
+ */
+#ifndef CADYCORE_H
+#define CADYCORE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ca_status {
+    CA_OK = 0,
+    CA_ERR_ARG = 1,        /* invalid argument (sizes, null pointers, unsupported option) */
+    CA_ERR_DEVICE = 2,     /* a HIP runtime call failed */
+    CA_ERR_NONFINITE = 3,  /* a state check found NaN or Inf */
+} ca_status;
+
+typedef struct ca_handle ca_handle;
+
+/* Geometry, factored into a horizontal part per stored column and a vertical
+ * part per level (no topography).  Column index = 16 * element + 4 * j + i.
+ * A Julia caller fills it from ClimaCore's LocalGeometry at one level and the
+ * face heights; cadycore/geometry.py builds it for a standalone run. */
+typedef struct ca_geometry {
+    int32_t nelem;          /* horizontal elements (0 => independent columns) */
+    int32_t ncol;           /* stored columns                                */
+    int32_t nlev;           /* centre levels                                 */
+    double radius;          /* deep-atmosphere radius (0 => no scaling)      */
+    const double *z_face;   /* [nlev + 1] face heights                       */
+    const double *J_h;      /* [ncol] horizontal Jacobian at radius          */
+    const double *gup;      /* [3 * ncol] g^11, g^12, g^22 at radius         */
+    const double *gdn;      /* [3 * ncol] g_11, g_12, g_22 at radius         */
+    const double *f3_num;   /* [ncol] 2 Omega sin(lat) (f-plane: f)          */
+    const double *f12;      /* [2 * ncol] contravariant horizontal Coriolis  */
+    const double *basis;    /* [12 * ncol] Cartesian a^1, a^2, a_1, a_2      */
+    int32_t n_dss;          /* number of shared-node groups                  */
+    const int32_t *dss_ptr; /* [n_dss + 1]                                   */
+    const int32_t *dss_col; /* [dss_ptr[n_dss]] member columns               */
+    const double *dss_w;    /* [dss_ptr[n_dss]] weights J / sum J            */
+    const double *gll_w;    /* [4] GLL weights                               */
+    const double *gll_D;    /* [16] D[i*4+k] = l_k'(xi_i)                    */
+} ca_geometry;
+
+/* Thermodynamic constants (ClimaParams names in cadycore/ca_params.toml). */
+typedef struct ca_params {
+    double R_d, cp_d, cv_d, T_0, p_ref_theta, grav, T_surf_ref, T_min_ref, T_min_sgs;
+} ca_params;
+
+/* Numerics of the dry parity profile. */
+typedef struct ca_config {
+    int32_t hyperdiff;      /* 1: Hyperdiffusion (sphere only)               */
+    double nu4_vort, nu4_scal, div_damp;
+    int32_t upwinding;      /* 0 none, 1 first_order, 3 vanleer_limiter      */
+    int32_t rayleigh;       /* 1: RayleighSponge                             */
+    double zd_rayleigh, alpha_uh, alpha_w;
+    const double *alpha_w_col; /* optional [ncol] per-column alpha_w        */
+    int32_t vert_diff;      /* 1: DecayWithHeightDiffusion(D0 per column, H = Inf) */
+    const double *D0_col;   /* [ncol] when vert_diff                         */
+    double dt;              /* p.dt (van Leer Courant number)                */
+} ca_config;
+
+/* Strided layout of one component: element (level v, node i, j, element h)
+ * lives at base[offset + v*s_v + i*s_i + j*s_j + h*s_h] (float units). */
+typedef struct ca_layout {
+    int64_t offset, s_v, s_i, s_j, s_h;
+} ca_layout;
+
+ca_status ca_create(const ca_geometry *geom, const ca_params *params, const ca_config *config,
+                    ca_handle **out);
+ca_status ca_destroy(ca_handle *h);
+int64_t ca_state_size(const ca_handle *h);
+const char *ca_last_error(const ca_handle *h);
+
+/* hook-level entry points (device pointers, stream-ordered) */
+ca_status ca_set_precomputed(ca_handle *h, float *Y, void *stream);
+ca_status ca_remaining_tendency(ca_handle *h, float *Yt, const float *Y, void *stream);
+ca_status ca_implicit_tendency(ca_handle *h, float *Yt, const float *Y, void *stream);
+ca_status ca_update_jacobian(ca_handle *h, const float *Y, double dtgamma, void *stream);
+ca_status ca_ldiv(ca_handle *h, float *dY, const float *R, void *stream);
+ca_status ca_correct_implicit_advection(ca_handle *h, float *Yt, const float *Y, void *stream);
+ca_status ca_dss(ca_handle *h, float *Y, void *stream);
+
+/* fused ARS343 step(s): Y <- step^nsteps(Y) */
+ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream);
+
+/* layout conversion: 5 components (rho, u_1, u_2, rho e_tot on nlev levels;
+ * u_3 on nlev + 1 levels) between strided caller arrays and native Y */
+ca_status ca_pack_state(ca_handle *h, const float *src, const ca_layout *layouts5, float *Y,
+                        void *stream);
+ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *layouts5, float *dst,
+                          void *stream);
+
+/* non-finite check (NaN poisoning counterpart): *flag = 1 if Y holds NaN/Inf.
+ * Synchronises the stream. */
+ca_status ca_check_finite(ca_handle *h, const float *Y, int32_t *flag, void *stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every kernel launch (off by default; enabling adds event overhead).
+ * ca_profile_get returns, for kernel class `id` in [0, ca_profile_count()),
+ * its name, the number of launches and the summed device time since the last
+ * ca_profile_enable(h, 1).  Synchronises the device. */
+ca_status ca_profile_enable(ca_handle *h, int32_t enable);
+int32_t ca_profile_count(void);
+ca_status ca_profile_get(ca_handle *h, int32_t id, const char **name, int64_t *launches,
+                         double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the native dry-dycore step on one MI355X.
+
+Workload (the headline configuration): dry global cubed sphere, h_elem = 30,
+Nq = 4, z_elem = 63 stretched to 30 km (dz_bottom = 30 m), Float32, dt = 60 s,
+balanced profile + zonal jet + seeded 1 K temperature noise below 5 km.
+A step is one ARS343 IMEX step (4 explicit tendencies with hyperdiffusion and
+DSS, 3 implicit Newton stages, upwind correction) on the whole sphere.
+
+Prints ONE JSON line.  ``value`` is simulated years per wall-clock day (SYPD)
+of the timed steps with the state resident in HBM.  ``roofline`` is for the
+kernel that takes the most device time, from HIP events recorded on the
+stream it is launched on.  ``cpu_baseline`` times the CPU restatement
+(``oracle/``) on the host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "climaatmos.jl_amd"))
+
+METRIC = "SYPD + achieved HBM GB/s, dry global h30 z63 Nq4 FP32; adjoint/forward ratio"
+HBM_PEAK_GBPS = 8000.0
+# bytes per state float moved by an ideal fused ARS343 step (compulsory
+# traffic; 58.2 state sizes per step)
+B_ALG_STATES_PER_STEP = 58.2
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--h-elem", type=int, default=30)
+    ap.add_argument("--z-elem", type=int, default=63)
+    ap.add_argument("--dt", type=float, default=60.0)
+    ap.add_argument("--cpu-baseline-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def kernel_bytes_newton(S, nC, nF):
+    """Algorithmic bytes of the three k_newton_stage launches of one step:
+    stage i reads u, i explicit tendencies and i-1 implicit tendencies (rho,
+    rho e_tot, u_3 planes) and writes U and F_I."""
+    fi = 2 * nC + nF
+    tot = 0
+    for i in (1, 2, 3):
+        tot += (S + i * S + (i - 1) * fi) + (S + fi)
+    return 4 * tot / 3.0
+
+
+def kernel_bytes_explicit(S, nC):
+    """k_explicit reads the state and the 4 DSSed Laplacian planes and writes
+    the tendency."""
+    return 4 * (S + 4 * nC + S)
+
+
+def read_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(grid, params, numerics, Y0, dt, nsteps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle  # the CPU restatement: timing baseline only
+
+    import numpy as np
+
+    o = Oracle(grid, params, numerics)
+    Y = np.asarray(Y0, dtype=np.float64).copy()
+    o.dss(Y)
+    t0 = time.perf_counter()
+    o.step(Y, dt, nsteps)
+    el = time.perf_counter() - t0
+    per_step = el / nsteps
+    return {
+        "value": (dt / per_step) / 365.0,
+        "unit": "SYPD",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{nsteps} full steps of the same h{grid.h_elem} z{grid.nlev} workload, Float64, 1 thread "
+                  f"({per_step:.2f} s/step)",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding is not built",
+              file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    from cadycore import geometry as G
+    from cadycore import initial as I
+    from cadycore import params as P
+    from cadycore.lib import DryDycore
+
+    par = P.Params.from_toml()
+    zf = G.tanh_stretched_faces(a.z_elem, 30000.0, 30.0)
+    grid = G.cubed_sphere(a.h_elem, zf, par.planet_radius, par.Omega)
+    num = P.Numerics(dt=a.dt)
+    Y0 = I.sphere_state(grid, par, seed=2)
+    dyc = DryDycore(grid, par, num)
+    S = dyc.S
+    nC, nF = grid.nlev * grid.ncol, (grid.nlev + 1) * grid.ncol
+    Y = dyc.state(Y0)
+    stream = torch.cuda.current_stream()
+    if a.warmup:
+        dyc.step(Y, a.dt, a.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dyc.step(Y, a.dt, a.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ok = dyc.check_finite(Y)
+    ms = el / a.steps * 1e3
+    sypd = (a.dt / (el / a.steps)) / 365.0
+    b_alg = B_ALG_STATES_PER_STEP * S * 4
+    step_gbps = b_alg / (el / a.steps) / 1e9
+
+    # per-kernel device time with HIP events on the launch stream (separate, untimed pass)
+    nprof = min(20, max(2, a.steps // 10))
+    dyc.profile_enable(True)
+    dyc.step(Y, a.dt, nprof)
+    prof = dyc.profile()
+    dyc.profile_enable(False)
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_n, dom_ms) = dom
+    avg_s = dom_ms / dom_n / 1e3
+    if dom_name == "k_newton_stage":
+        per_launch = kernel_bytes_newton(S, nC, nF)
+    elif dom_name == "k_explicit":
+        per_launch = kernel_bytes_explicit(S, nC)
+    else:
+        per_launch = None
+    achieved = per_launch / avg_s / 1e9 if per_launch else None
+    traffic = read_traffic(dom_name)
+    roof = {
+        "kernel": dom_name,
+        "bound": "hbm",
+        "achieved": round(achieved, 1) if achieved else None,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
+        "traffic": traffic,
+        "avg_launch_us": round(avg_s * 1e6, 2),
+        "alg_bytes_per_launch": per_launch,
+    }
+    kernels = {k: {"launches": n, "ms_per_step": round(t / nprof, 4)} for k, (n, t) in prof.items() if n}
+
+    cpu = None
+    if not a.no_cpu_baseline and a.cpu_baseline_steps > 0:
+        cpu = cpu_baseline(grid, par, num, Y0, a.dt, a.cpu_baseline_steps)
+
+    line = {
+        "metric": METRIC,
+        "value": round(sypd, 3),
+        "unit": "SYPD",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded balanced profile + jet + noise)",
+        "config": {
+            "workload": f"dry global cubed sphere h_elem={a.h_elem} Nq=4 z_elem={a.z_elem} (30 km, dz_bottom 30 m), "
+                        f"FP32, ARS343 dt={a.dt:g}s",
+            "columns": grid.ncol,
+            "state_floats": S,
+        },
+        "achieved_step_GBps": round(step_gbps, 1),
+        "step_roofline_frac": round(step_gbps / HBM_PEAK_GBPS, 4),
+        "steps_per_s": round(1e3 / ms, 2),
+        "adjoint_forward_ratio": None,
+        "finite": bool(ok),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

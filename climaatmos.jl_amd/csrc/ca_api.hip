@@ -270,6 +270,14 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         lf[3 * (size_t)nf + f] = (float)(f >= 1 && f < nl ? par->grav * (zcd[f] - zcd[f - 1]) : 0.0);
         lf[4 * (size_t)nf + f] = (float)zeta(zf[f]);
     }
+    std::vector<double> dphid(nl + 1, 0.0), phicd(nl);
+    for (int f = 1; f < nl; ++f) dphid[f] = par->grav * (zcd[f] - zcd[f - 1]);
+    for (int k = 0; k < nl; ++k) phicd[k] = par->grav * zcd[k];
+    double *ddphi = nullptr, *dphic = nullptr;
+    if ((s = upload(h, &ddphi, dphid))) return s;
+    if ((s = upload(h, &dphic, phicd))) return s;
+    g.dPhif_d = ddphi;
+    g.Phic_d = dphic;
     float *dlc = nullptr, *dlf = nullptr;
     if ((s = upload(h, &dlc, lc))) return s;
     if ((s = upload(h, &dlf, lf))) return s;
@@ -297,22 +305,30 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         g.D0_col = p;
     }
     g.ndss = geom->nelem > 0 ? geom->n_dss : 0;
+    g.col_group = nullptr;
     if (g.ndss > 0) {
         int m = geom->dss_ptr[g.ndss];
         std::vector<int> ptr(geom->dss_ptr, geom->dss_ptr + g.ndss + 1), cols(geom->dss_col, geom->dss_col + m);
-        for (int c : cols)
-            if (c < 0 || c >= nc) {
-                delete h;
-                return CA_ERR_ARG;
+        std::vector<int> cgrp(nc, -1);
+        for (int gi = 0; gi < g.ndss; ++gi)
+            for (int t = ptr[gi]; t < ptr[gi + 1]; ++t) {
+                int c = cols[t];
+                if (c < 0 || c >= nc || cgrp[c] != -1) {
+                    delete h;
+                    return CA_ERR_ARG;
+                }
+                cgrp[c] = gi;
             }
-        int *dp = nullptr, *dcl = nullptr;
+        int *dp = nullptr, *dcl = nullptr, *dg = nullptr;
         float *dw = nullptr;
         if ((s = upload(h, &dp, ptr))) return s;
         if ((s = upload(h, &dcl, cols))) return s;
         if ((s = upload(h, &dw, tofloat(geom->dss_w, m)))) return s;
+        if ((s = upload(h, &dg, cgrp))) return s;
         g.dptr = dp;
         g.dcol = dcl;
         g.dw = dw;
+        g.col_group = dg;
     }
     float W[16], Dm[4][4];
     for (int j = 0; j < 4; ++j)
@@ -333,6 +349,15 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     n.Tsr = (float)par->T_surf_ref;
     n.Tmr = (float)par->T_min_ref;
     n.Tmin = (float)par->T_min_sgs;
+    n.dR_d = par->R_d;
+    n.dcp_d = par->cp_d;
+    n.dcv_d = par->cv_d;
+    n.dT_0 = par->T_0;
+    n.dp0inv = 1.0 / par->p_ref_theta;
+    n.dkex = par->R_d / par->cp_d;
+    n.dTsr = par->T_surf_ref;
+    n.dTmr = par->T_min_ref;
+    n.dTmin = par->T_min_sgs;
     n.hyper = cfg->hyperdiff && geom->nelem > 0;
     n.nu_v = (float)cfg->nu4_vort;
     n.nu_s = (float)cfg->nu4_scal;
@@ -449,16 +474,16 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
     ca_status s;
-    // the step assumes a continuous, filtered state (the reference's state
-    // after dss! and cache! at the end of every step)
-    if ((s = dss_state(h, Y, st, KID_DSS_STATE))) return s;
+    // the step starts from a continuous, filtered state (the reference's
+    // state after dss! and cache! at the end of every step): a zero-increment
+    // pass of the fused combination + DSS kernel establishes it
     {
         FinalSrc fs{};
         fs.nfe = 0;
         fs.nfi = 0;
         fs.dt = 0.0f;
         Launch l(h, st, KID_FINAL);
-        k_final<<<cdiv(h->S, 256), 256, 0, st>>>(g, Y, fs);
+        k_final<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, Y, fs);
     }
     for (int stp = 0; stp < nsteps; ++stp) {
         // stage 1: explicit tendency at u
@@ -506,7 +531,7 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
             }
         {
             Launch l(h, st, KID_FINAL);
-            k_final<<<cdiv(h->S, 256), 256, 0, st>>>(g, Y, fs);
+            k_final<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, Y, fs);
         }
         if ((s = check_launch(h, "k_final"))) return s;
     }

@@ -36,12 +36,21 @@ __device__ __forceinline__ CV cv_make(const DevGeom &g, const DevNum &n, int k, 
     float gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
     float vv = 0.5f * (u3a * ga * u3a + u3b * gb * u3b);
     c.K = 0.5f * ((u1 * ct1 + u2 * ct2) + vv);
-    float e_tot = rhoe / rho;
-    float e_int = e_tot - c.K - g.Phic[k];
-    c.T = ca_temperature(n, e_int);
-    c.h = e_tot + n.R_d * c.T;
-    c.p = rho * n.R_d * c.T;
-    c.Pi = ca_exner(n, c.p);
+    // thermodynamics in float64: the implicit pressure gradient is a small
+    // difference of large hydrostatic terms, so T and p are kept to double
+    // precision for it (the state itself stays float32)
+    double e_tot = (double)rhoe / (double)rho;
+    double e_int = e_tot - (double)c.K - g.Phic_d[k];
+    double Td = n.dT_0 + (e_int + n.dR_d * n.dT_0) / n.dcv_d;
+    Td = Td > n.dTmin ? Td : n.dTmin;
+    double pd = (double)rho * n.dR_d * Td;
+    c.Td = Td;
+    c.pd = pd;
+    c.Pid = exp(n.dkex * log(pd * n.dp0inv));
+    c.T = (float)Td;
+    c.h = (float)(e_tot + n.dR_d * Td);
+    c.p = (float)pd;
+    c.Pi = (float)c.Pid;
     c.J = g.Jh[col] * s * s * g.Jvc[k];
     return c;
 }
@@ -70,11 +79,16 @@ __device__ __forceinline__ float upwind_product(int scheme, int f, int nlev, flo
 // above (b): -(grad Phi - grad Phi_r + cp_d I^f(theta_v - theta_vr) grad Pi)
 // (implicit_tendency.jl:287-293), written with cancellation-free differences.
 __device__ __forceinline__ float pgf_u3(const DevGeom &g, const DevNum &n, int f, const CV &a, const CV &b) {
-    float dPhr = ca_dPhiref(n, a.p, a.Pi, b.p);
-    float dPi = ca_dExner(n, a.p, a.Pi, b.p);
-    float tha = (a.T - ca_Tref(n, a.Pi)) / a.Pi;
-    float thb = (b.T - ca_Tref(n, b.Pi)) / b.Pi;
-    return -((g.dPhif[f] - dPhr) + n.cp_d * (0.5f * (tha + thb)) * dPi);
+    double lr = n.dkex * log1p((b.pd - a.pd) / a.pd);
+    double Pa = a.Pid;
+    double Pa7 = Pa * Pa * Pa * Pa * Pa * Pa * Pa;
+    double dPi = Pa * expm1(lr);
+    double dPhr = -n.dcp_d * (n.dTmr * lr + (n.dTsr - n.dTmr) * (1.0 / 7.0) * Pa7 * expm1(7.0 * lr));
+    double Pb = Pa + dPi;
+    double Pb7 = Pb * Pb * Pb * Pb * Pb * Pb * Pb;
+    double tha = (a.Td - (n.dTmr + (n.dTsr - n.dTmr) * Pa7)) / Pa;
+    double thb = (b.Td - (n.dTmr + (n.dTsr - n.dTmr) * Pb7)) / Pb;
+    return (float)(-((g.dPhif_d[f] - dPhr) + n.dcp_d * (0.5 * (tha + thb)) * dPi));
 }
 
 __device__ __forceinline__ float beta_w(const DevGeom &g, const DevNum &n, int f, int col) {
@@ -300,12 +314,14 @@ __global__ void k_correct(DevGeom g, DevNum n, float *Yt, const float *Y) {
     const int nl = g.nlev;
     const float *u3 = Y + 4 * g.nC;
     float hv[6], rJv[6];
-    // centres k-3 .. k+2 (as available)
+    // centres k-3 .. k+2 (as available); h relative to the lowest cell (the
+    // shift cancels between the upwind and central products)
+    const float href = cv_load(g, n, Y, 0, col).h;
     for (int t = 0; t < 6; ++t) {
         int kk = k - 3 + t;
         if (kk >= 0 && kk < nl) {
             CV c = cv_load(g, n, Y, kk, col);
-            hv[t] = c.h;
+            hv[t] = c.h - href;
             rJv[t] = c.rho * c.J;
         } else {
             hv[t] = 0.0f;
@@ -475,13 +491,18 @@ __global__ void __launch_bounds__(128) k_newton_stage(DevGeom g, DevNum n, Stage
             rw[t] = 0.0f;
         }
         // preload centres 0 and 1 into slots 2, 3 (face 0 is a boundary)
+        // h is taken relative to its value in the lowest cell: both the upwind
+        // and the central product shift by the same u^3 h_ref, which cancels in
+        // their difference, and the float32 differences stay accurate
+        float href;
         {
             CV c = cv_load(g, n, U, 0, col);
-            hw[2] = c.h;
+            href = c.h;
+            hw[2] = 0.0f;
             rw[2] = c.rho * c.J;
             if (nl > 1) {
                 CV d1 = cv_load(g, n, U, 1, col);
-                hw[3] = d1.h;
+                hw[3] = d1.h - href;
                 rw[3] = d1.rho * d1.J;
             }
         }
@@ -496,7 +517,7 @@ __global__ void __launch_bounds__(128) k_newton_stage(DevGeom g, DevNum n, Stage
             rw[2] = rw[3];
             if (f + 1 <= nl - 1) {
                 CV c = cv_load(g, n, U, f + 1, col);
-                hw[3] = c.h;
+                hw[3] = c.h - href;
                 rw[3] = c.rho * c.J;
             } else {
                 hw[3] = 0.0f;
@@ -527,23 +548,75 @@ __global__ void __launch_bounds__(128) k_newton_stage(DevGeom g, DevNum n, Stage
     }
 }
 
-// Final combination u <- u + dt sum_j b_j (F_E,j + F_I,j); boundary u_3 = 0.
-__global__ void k_final(DevGeom g, float *u, FinalSrc s) {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t S = 4 * g.nC + g.nF;
-    if (idx >= S) return;
+// Final combination u <- u + dt sum_j b_j (F_E,j + F_I,j), fused with the
+// end-of-step dss! (integrator.jl:217-223): a shared node's new value is the
+// J-weighted average over its copies, written identically to every copy, so
+// duplicated element-edge columns stay bitwise equal from step to step.
+// Boundary u_3 = 0 (impenetrability filter of the end-of-step cache!).
+// Thread per (column, level y): y < nlev handles the four centre components
+// of level y, every y the u_3 face y.  Only the first copy of a shared node
+// (the group leader) works for the group.
+__device__ __forceinline__ float comb_center(const FinalSrc &s, const float *u, int comp, int64_t q, int64_t nC) {
     float acc = 0.0f;
-    for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][idx], acc);
-    int comp = idx < 4 * g.nC ? (int)(idx / g.nC) : 4;
-    int64_t r = idx - (int64_t)comp * g.nC;
-    if (comp == 0 || comp == 3 || comp == 4) {
-        int ci = comp == 0 ? 0 : (comp == 3 ? 1 : 2);
-        for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][ci * g.nC + r], acc);
+    for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][comp * nC + q], acc);
+    if (comp == 0 || comp == 3) {
+        int ci = comp == 0 ? 0 : 1;
+        for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][ci * nC + q], acc);
     }
-    float v = u[idx] + s.dt * acc;
-    if (comp == 4) {
-        int f = (int)(r / g.ncol);
-        if (f == 0 || f == g.nlev) v = 0.0f;
+    return u[comp * nC + q] + s.dt * acc;
+}
+__device__ __forceinline__ float comb_face(const FinalSrc &s, const float *u, int64_t q, int64_t nC) {
+    float acc = 0.0f;
+    for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][4 * nC + q], acc);
+    for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][2 * nC + q], acc);
+    return u[4 * nC + q] + s.dt * acc;
+}
+
+__global__ void __launch_bounds__(256) k_final(DevGeom g, float *u, FinalSrc s) {
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (col >= g.ncol) return;
+    const int64_t nC = g.nC, nc = g.ncol;
+    const int grp = g.col_group ? g.col_group[col] : -1;
+    const bool bnd_face = (y == 0 || y == g.nlev);
+    if (grp < 0) {
+        int64_t q = (int64_t)y * nc + col;
+        float v[4];
+        if (y < g.nlev)
+            for (int comp = 0; comp < 4; ++comp) v[comp] = comb_center(s, u, comp, q, nC);
+        float w = bnd_face ? 0.0f : comb_face(s, u, q, nC);
+        if (y < g.nlev)
+            for (int comp = 0; comp < 4; ++comp) u[comp * nC + q] = v[comp];
+        u[4 * nC + q] = w;
+        return;
     }
-    u[idx] = v;
+    const int b = g.dptr[grp], e = g.dptr[grp + 1];
+    if (g.dcol[b] != col) return;  // not the leader
+    const float *B = g.basis;
+    float sr = 0.0f, se = 0.0f, sw = 0.0f, X0 = 0.0f, X1 = 0.0f, X2 = 0.0f;
+    for (int t = b; t < e; ++t) {
+        const int c = g.dcol[t];
+        const float wt = g.dw[t];
+        const int64_t q = (int64_t)y * nc + c;
+        if (y < g.nlev) {
+            sr = fmaf(wt, comb_center(s, u, 0, q, nC), sr);
+            se = fmaf(wt, comb_center(s, u, 3, q, nC), se);
+            float a = wt * comb_center(s, u, 1, q, nC), bb = wt * comb_center(s, u, 2, q, nC);
+            X0 += a * B[0 * nc + c] + bb * B[3 * nc + c];
+            X1 += a * B[1 * nc + c] + bb * B[4 * nc + c];
+            X2 += a * B[2 * nc + c] + bb * B[5 * nc + c];
+        }
+        if (!bnd_face) sw = fmaf(wt, comb_face(s, u, q, nC), sw);
+    }
+    for (int t = b; t < e; ++t) {
+        const int c = g.dcol[t];
+        const int64_t q = (int64_t)y * nc + c;
+        if (y < g.nlev) {
+            u[q] = sr;
+            u[3 * nC + q] = se;
+            u[nC + q] = X0 * B[6 * nc + c] + X1 * B[7 * nc + c] + X2 * B[8 * nc + c];
+            u[2 * nC + q] = X0 * B[9 * nc + c] + X1 * B[10 * nc + c] + X2 * B[11 * nc + c];
+        }
+        u[4 * nC + q] = bnd_face ? 0.0f : sw;
+    }
 }

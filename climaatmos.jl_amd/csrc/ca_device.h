@@ -27,6 +27,9 @@ struct DevGeom {
     const float *zf, *Jvf, *sf, *dPhif;  // faces [nlev + 1]; dPhif[f] = g (zc[f] - zc[f-1])
     const float *zeta_c, *zeta_f;        // Rayleigh profiles sin^2(...) (0 below zd)
     const float *Phic;                   // g zc
+    const double *dPhif_d;               // dPhif in float64
+    const double *Phic_d;                // g zc in float64
+    const int *col_group;                // shared-node group of a column (-1: interior node)
     const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
     // DSS groups
     int ndss;
@@ -36,6 +39,7 @@ struct DevGeom {
 
 struct DevNum {
     float R_d, cp_d, cv_d, T_0, p0inv, kex, kap, grav, Tsr, Tmr, Tmin;
+    double dR_d, dcp_d, dcv_d, dT_0, dp0inv, dkex, dTsr, dTmr, dTmin;
     float nu_v, nu_s, dd;
     float alpha_uh, alpha_w, dt;
     int hyper, upw, rayleigh, vdiff, sphere;

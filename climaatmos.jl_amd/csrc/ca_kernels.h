@@ -7,6 +7,7 @@ struct CV {
     float rho, u1, u2, rhoe;
     float ct1, ct2;  // contravariant horizontal velocity
     float K, T, h, p, Pi, J;
+    double Td, pd, Pid;  // temperature, pressure and Exner function in float64
 };
 
 // Jacobian bands (update_jacobian! storage for the hook-level ldiv)

@@ -39,26 +39,32 @@ def parse():
     ap.add_argument("--h-elem", type=int, default=30)
     ap.add_argument("--z-elem", type=int, default=63)
     ap.add_argument("--dt", type=float, default=60.0)
-    ap.add_argument("--cpu-baseline-steps", type=int, default=2)
+    ap.add_argument("--cpu-baseline-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
+                         "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
     return ap.parse_args()
 
 
-def kernel_bytes_newton(S, nC, nF):
-    """Algorithmic bytes of the three k_newton_stage launches of one step:
-    stage i reads u, i explicit tendencies and i-1 implicit tendencies (rho,
-    rho e_tot, u_3 planes) and writes U and F_I."""
+def kernel_bytes(S, nC, nF):
+    """Algorithmic (compulsory) bytes per launch of each kernel of the step,
+    keyed by the profile names of the library.  fi = one implicit tendency
+    (rho, rho e_tot and u_3 planes).  Newton stage i (i = 1, 2, 3) reads u,
+    i explicit and i - 1 implicit tendencies and writes U and F_I; the three
+    stages are averaged, as their launches are.  The final combination reads
+    u, 4 explicit and 3 implicit tendencies and writes u."""
     fi = 2 * nC + nF
-    tot = 0
-    for i in (1, 2, 3):
-        tot += (S + i * S + (i - 1) * fi) + (S + fi)
-    return 4 * tot / 3.0
-
-
-def kernel_bytes_explicit(S, nC):
-    """k_explicit reads the state and the 4 DSSed Laplacian planes and writes
-    the tendency."""
-    return 4 * (S + 4 * nC + S)
+    newton = sum((S + i * S + (i - 1) * fi) + (S + fi) for i in (1, 2, 3)) / 3.0
+    return {
+        "k_laplacians": 4 * (S + 4 * nC),
+        "k_dss(laplacians)": 4 * (2 * 4 * nC),
+        "k_explicit": 4 * (S + 4 * nC + S),
+        "k_dss(tendency)": 4 * (2 * S),
+        "k_newton_stage": 4 * newton,
+        "k_final": 4 * (S + 4 * S + 3 * fi + S),
+        "k_dss(state)": 4 * (2 * S),
+    }
 
 
 def read_traffic(kernel):
@@ -77,27 +83,50 @@ def read_traffic(kernel):
         return None
 
 
+def host_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(grid, params, numerics, Y0, dt, nsteps):
+    """The CPU restatement (oracle/, Float32 build, OpenMP over columns,
+    elements and node groups) stepping the same workload on the host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle  # the CPU restatement: timing baseline only
 
     import numpy as np
 
-    o = Oracle(grid, params, numerics)
-    Y = np.asarray(Y0, dtype=np.float64).copy()
+    o = Oracle(grid, params, numerics, f32=True)
+    Y = np.asarray(Y0, dtype=np.float32).copy()
     o.dss(Y)
+    o.step(Y, dt, 1)  # first touch of the scratch arrays, untimed
     t0 = time.perf_counter()
     o.step(Y, dt, nsteps)
     el = time.perf_counter() - t0
     per_step = el / nsteps
+    threads = host_threads()
     return {
-        "value": (dt / per_step) / 365.0,
+        "value": round((dt / per_step) / 365.0, 5),
         "unit": "SYPD",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{nsteps} full steps of the same h{grid.h_elem} z{grid.nlev} workload, Float64, 1 thread "
-                  f"({per_step:.2f} s/step)",
+        "sample": f"{nsteps} full ARS343 steps of the same h{grid.h_elem} z{grid.nlev} workload (after one untimed "
+                  f"step), Float32, {threads} OpenMP threads: {per_step:.3f} s/step",
     }
+
+
+def copy_calibration(torch):
+    """Three launches of a plain device copy of 256 MiB (read 256 MiB, write
+    256 MiB): the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -146,12 +175,8 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     dom_name, (dom_n, dom_ms) = dom
     avg_s = dom_ms / dom_n / 1e3
-    if dom_name == "k_newton_stage":
-        per_launch = kernel_bytes_newton(S, nC, nF)
-    elif dom_name == "k_explicit":
-        per_launch = kernel_bytes_explicit(S, nC)
-    else:
-        per_launch = None
+    kb = kernel_bytes(S, nC, nF)
+    per_launch = kb.get(dom_name)
     achieved = per_launch / avg_s / 1e9 if per_launch else None
     traffic = read_traffic(dom_name)
     roof = {
@@ -165,7 +190,17 @@ def main():
         "avg_launch_us": round(avg_s * 1e6, 2),
         "alg_bytes_per_launch": per_launch,
     }
-    kernels = {k: {"launches": n, "ms_per_step": round(t / nprof, 4)} for k, (n, t) in prof.items() if n}
+    kernels = {}
+    for k, (n, t) in prof.items():
+        if not n:
+            continue
+        kernels[k] = {"launches_per_step": n / nprof, "us_per_launch": round(t / n * 1e3, 2),
+                      "ms_per_step": round(t / nprof, 4)}
+        if k in kb:
+            kernels[k]["alg_GBps"] = round(kb[k] / (t / n / 1e3) / 1e9, 1)
+
+    if a.copy_calibration:
+        copy_calibration(torch)
 
     cpu = None
     if not a.no_cpu_baseline and a.cpu_baseline_steps > 0:

@@ -66,7 +66,9 @@ EXPORTED = (
     "ca_remaining_tendency", "ca_implicit_tendency", "ca_update_jacobian", "ca_ldiv",
     "ca_correct_implicit_advection", "ca_dss", "ca_step", "ca_pack_state", "ca_unpack_state",
     "ca_check_finite", "ca_profile_enable", "ca_profile_count", "ca_profile_get",
+    "ca_set_tuning", "ca_get_tuning",
 )
+CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK = 0, 1, 2
 
 
 def load_library(path: str = LIB_PATH):
@@ -91,6 +93,8 @@ def load_library(path: str = LIB_PATH):
     lib.ca_check_finite.argtypes = [vp, vp, C.POINTER(C.c_int32), vp]
     lib.ca_profile_enable.argtypes = [vp, C.c_int32]
     lib.ca_profile_count.argtypes = []
+    lib.ca_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.ca_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     lib.ca_profile_get.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_double)]
     for nm in EXPORTED:
@@ -113,7 +117,7 @@ class DryDycore:
     """
 
     def __init__(self, grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None, D0_col=None,
-                 device=None):
+                 device=None, lib_path: str = LIB_PATH):
         import torch
 
         if not torch.cuda.is_available():
@@ -121,7 +125,7 @@ class DryDycore:
         self.torch = torch
         self.device = torch.device(device or "cuda:0")
         torch.cuda.set_device(self.device)
-        self.lib = load_library()
+        self.lib = load_library(lib_path)
         self.grid, self.params, self.numerics = grid, params, numerics
         keep = {}
 
@@ -236,6 +240,14 @@ class DryDycore:
         flag = C.c_int32(0)
         st = self.lib.ca_check_finite(self.h, self._ptr(Y), C.byref(flag), self._stream())
         return st == CA_OK and flag.value == 0
+
+    def set_tuning(self, knob, value):
+        self._check(self.lib.ca_set_tuning(self.h, int(knob), int(value)), "set_tuning")
+
+    def get_tuning(self, knob):
+        v = C.c_int32()
+        self._check(self.lib.ca_get_tuning(self.h, int(knob), C.byref(v)), "get_tuning")
+        return v.value
 
     # ------------------------------------------------------------ profiling
     def profile_enable(self, on=True):

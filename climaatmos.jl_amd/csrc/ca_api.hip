@@ -46,6 +46,7 @@ struct ca_handle {
     float *FE[4] = {nullptr, nullptr, nullptr, nullptr};
     float *FI[4] = {nullptr, nullptr, nullptr, nullptr};
     float *U = nullptr;
+    float *raw = nullptr;   // un-DSSed tendency / Laplacians (gather-DSS source)
     float *bands = nullptr;
     Bands B{};
     int *flag = nullptr;
@@ -53,6 +54,9 @@ struct ca_handle {
     bool prof = false;
     std::vector<ProfRec> recs;
     double aE[4][4], aI[4][4], b[4];
+    int newton_variant = 1;
+    int explicit_chunk = 16;
+    int dss_chunk = 2;
 };
 
 namespace {
@@ -143,39 +147,101 @@ ca_status check_launch(ca_handle *h, const char *what) {
     return CA_OK;
 }
 
-ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
-    if (h->g.nelem == 0 || h->g.ndss == 0) return CA_OK;
-    const DevGeom &g = h->g;
-    {
-        Launch l(h, st, kid);
-        // rho and rho e_tot planes
-        k_dss_scalar<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, Y, 2, g.nlev, 3 * g.nC);
-        k_dss_vector<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, Y + g.nC, Y + 2 * g.nC, g.nlev);
-        k_dss_scalar<<<dim3(cdiv(g.ndss, 256), g.nlev + 1), 256, 0, st>>>(g, Y + 4 * g.nC, 1, g.nlev + 1, 0);
-    }
-    return check_launch(h, "dss");
+// 1-D grid of (column block, level chunk) pairs for k_dss_gather
+// k_dss_gather remaps block b to logical block (b % 8) * q + b / 8 with
+// q = ceil(nb / 8): a bijection onto [0, 8 q) only if the grid holds 8 q
+// blocks; logical blocks past nb return at once.
+inline unsigned dss_grid(const DevGeom &g, int chunk) {
+    const unsigned nb = cdiv(g.ncol, 256) * cdiv(g.nlev, chunk);
+    return 8 * cdiv(nb, 8);
 }
 
-// remaining_tendency!: Laplacians, their DSS, the fused explicit kernel
-ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st) {
+// copy src -> dst with the impenetrability filter (columns without DSS)
+ca_status filter_columns(ca_handle *h, const float *src, float *dst, hipStream_t st) {
+    const DevGeom &g = h->g;
+    k_dss_gather<<<dss_grid(g, g.nlev), 256, 0, st>>>(g, src, dst, 4, -1, 1, 1, g.nlev);
+    return check_launch(h, "filter");
+}
+
+// gather DSS of a full state-shaped buffer, src -> dst (out of place)
+ca_status dss_gather_state(ca_handle *h, const float *src, float *dst, int filter, hipStream_t st, int kid) {
+    const DevGeom &g = h->g;
+    const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
+    {
+        Launch l(h, st, kid);
+        k_dss_gather<<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, 4, 1, 1, filter, ch);
+    }
+    return check_launch(h, "k_dss_gather(state)");
+}
+
+// in-place state DSS (dss! hook): copy to the scratch buffer, gather back
+ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
+    if (h->g.nelem == 0 || h->g.ndss == 0) return CA_OK;
+    HIPCHK(h, hipMemcpyAsync(h->raw, Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return dss_gather_state(h, h->raw, Y, 0, st, kid);
+}
+
+// remaining_tendency!: Laplacians, their DSS, the fused explicit kernel.
+// With dss_out, the tendency is written un-DSSed to the scratch buffer and
+// gathered (DSSed) into Yt; otherwise it goes to Yt as the reference's
+// remaining_tendency! returns it.
+ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st, bool dss_out = false) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
+    const bool dss = n.sphere && g.ndss > 0;
     if (n.sphere && n.hyper) {
         {
             Launch l(h, st, KID_LAPLACIANS);
-            k_laplacians<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, n, Y, h->L);
+            k_laplacians<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, n, Y, h->raw);
         }
         {
             Launch l(h, st, KID_DSS_LAPLACIAN);
-            k_dss_vector<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, h->L, h->L + g.nC, g.nlev);
-            k_dss_scalar<<<dim3(cdiv(g.ndss, 256), g.nlev), 256, 0, st>>>(g, h->L + 2 * g.nC, 2, g.nlev, g.nC);
+            const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
+            k_dss_gather<<<dss_grid(g, ch), 256, 0, st>>>(g, h->raw, h->L, 4, 0, 0, 0, ch);
         }
     }
+    float *out = (dss_out && dss) ? h->raw : Yt;
     {
         Launch l(h, st, KID_EXPLICIT);
-        k_explicit<<<cdiv(g.ncol, 256), 256, 0, st>>>(g, n, Y, h->L, Yt);
+        const int ch = h->explicit_chunk;
+        const int nchunks = ch > 0 ? (int)cdiv(g.nlev, ch) : 1;
+        const dim3 grid(cdiv(g.ncol, 256), nchunks), blk(256);
+        const bool hy = n.hyper != 0, vd = n.vdiff != 0;
+        if (n.sphere) {
+            if (hy && !vd) k_explicit<true, true, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+            else if (!hy && !vd) k_explicit<true, false, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+            else if (hy && vd) k_explicit<true, true, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+            else k_explicit<true, false, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+        } else {
+            if (vd) k_explicit<false, false, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+            else k_explicit<false, false, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+        }
     }
-    return check_launch(h, "explicit tendency");
+    ca_status s = check_launch(h, "explicit tendency");
+    if (s) return s;
+    if (dss_out && dss) return dss_gather_state(h, h->raw, Yt, 0, st, KID_DSS_TENDENCY);
+    return CA_OK;
+}
+
+template <int TC>
+void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const dim3 grid(cdiv(g.ncol, TC)), blk(256);
+    if (src.nfull == 2 && src.nfi == 0)
+        k_newton_pcr<TC, 2, 0><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+    else if (src.nfull == 3 && src.nfi == 1)
+        k_newton_pcr<TC, 3, 1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+    else if (src.nfull == 4 && src.nfi == 2)
+        k_newton_pcr<TC, 4, 2><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+    else
+        k_newton_pcr<TC, -1, -1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+}
+
+void launch_newton_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, hipStream_t st) {
+    if (h->newton_variant == 2)
+        launch_pcr_tc<32>(h, src, dtg, i, st);
+    else
+        launch_pcr_tc<16>(h, src, dtg, i, st);
 }
 
 ca_status ensure_bands(ca_handle *h) {
@@ -329,6 +395,61 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         g.dcol = dcl;
         g.dw = dw;
         g.col_group = dg;
+        // per-column gather table (slot 0 = own column), computed in double
+        std::vector<int> gidx(4 * (size_t)nc), grank(nc, 0), gmv(nc, 1);
+        std::vector<float> gw(4 * (size_t)nc, 0.0f), gM(16 * (size_t)nc, 0.0f);
+        const double *Bs = geom->basis;
+        auto bas = [&](int plane, int c) { return Bs[(size_t)plane * nc + c]; };
+        for (int c = 0; c < nc; ++c) {
+            for (int t = 0; t < 4; ++t) gidx[(size_t)t * nc + c] = c;
+            gw[c] = 1.0f;
+            gM[0 * (size_t)nc + c] = 1.0f;
+            gM[3 * (size_t)nc + c] = 1.0f;
+            const int gi = cgrp[c];
+            if (gi < 0) continue;
+            const int b0 = ptr[gi], m = ptr[gi + 1] - b0;
+            if (m > 4) {
+                delete h;
+                return CA_ERR_ARG;
+            }
+            int slot = 1, rk = 0;
+            for (int r = 0; r < m; ++r) {
+                const int mcol = cols[b0 + r];
+                const int t = mcol == c ? 0 : slot++;
+                gidx[(size_t)t * nc + c] = mcol;
+                gw[(size_t)t * nc + c] = (float)geom->dss_w[b0 + r];
+                rk |= r << (2 * t);
+                const double w = geom->dss_w[b0 + r];
+                double m00 = 0, m01 = 0, m10 = 0, m11 = 0;
+                for (int d = 0; d < 3; ++d) {
+                    m00 += bas(6 + d, c) * bas(d, mcol);
+                    m01 += bas(6 + d, c) * bas(3 + d, mcol);
+                    m10 += bas(9 + d, c) * bas(d, mcol);
+                    m11 += bas(9 + d, c) * bas(3 + d, mcol);
+                }
+                gM[(size_t)(4 * t + 0) * nc + c] = (float)(w * m00);
+                gM[(size_t)(4 * t + 1) * nc + c] = (float)(w * m01);
+                gM[(size_t)(4 * t + 2) * nc + c] = (float)(w * m10);
+                gM[(size_t)(4 * t + 3) * nc + c] = (float)(w * m11);
+            }
+            grank[c] = rk;
+            gmv[c] = m;
+        }
+        int *d_gidx = nullptr, *d_grank = nullptr, *d_gm = nullptr;
+        float *d_gw = nullptr, *d_gM = nullptr;
+        if ((s = upload(h, &d_gidx, gidx))) return s;
+        if ((s = upload(h, &d_grank, grank))) return s;
+        if ((s = upload(h, &d_gm, gmv))) return s;
+        if ((s = upload(h, &d_gw, gw))) return s;
+        if ((s = upload(h, &d_gM, gM))) return s;
+        g.gidx = d_gidx;
+        g.grank = d_grank;
+        g.gm = d_gm;
+        g.gw = d_gw;
+        g.gM = d_gM;
+    } else {
+        g.gidx = g.grank = g.gm = nullptr;
+        g.gw = g.gM = nullptr;
     }
     float W[16], Dm[4][4];
     for (int j = 0; j < 4; ++j)
@@ -378,9 +499,11 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     for (int i = 1; i < 4; ++i)
         if ((s = dalloc(h, &h->FI[i], 2 * (size_t)g.nC + (size_t)g.nF))) return s;
     if ((s = dalloc(h, &h->U, (size_t)h->S))) return s;
+    if ((s = dalloc(h, &h->raw, (size_t)h->S))) return s;
     if ((s = dalloc(h, &h->flag, 1))) return s;
     HIPCHK(h, hipMemset(h->L, 0, 4 * (size_t)g.nC * sizeof(float)));
     ars343(h);
+    h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
     *out = h;
     return CA_OK;
 }
@@ -475,20 +598,21 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
     const DevGeom &g = h->g;
     ca_status s;
     // the step starts from a continuous, filtered state (the reference's
-    // state after dss! and cache! at the end of every step): a zero-increment
-    // pass of the fused combination + DSS kernel establishes it
-    {
+    // state after dss! and cache! at the end of every step)
+    if (g.nelem > 0 && g.ndss > 0) {
+        HIPCHK(h, hipMemcpyAsync(h->U, Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
+        if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
+    } else {
         FinalSrc fs{};
-        fs.nfe = 0;
-        fs.nfi = 0;
         fs.dt = 0.0f;
         Launch l(h, st, KID_FINAL);
-        k_final<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, Y, fs);
+        k_combine<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(g, Y, h->U, fs);
+        if ((s = check_launch(h, "k_combine"))) return s;
+        if ((s = filter_columns(h, h->U, Y, st))) return s;
     }
     for (int stp = 0; stp < nsteps; ++stp) {
-        // stage 1: explicit tendency at u
-        if ((s = explicit_tendency(h, h->FE[0], Y, st))) return s;
-        if ((s = dss_state(h, h->FE[0], st, KID_DSS_TENDENCY))) return s;
+        // stage 1: explicit tendency at u (DSSed)
+        if ((s = explicit_tendency(h, h->FE[0], Y, st, true))) return s;
         for (int i = 1; i < 4; ++i) {
             StageSrc src{};
             src.full[0] = Y;
@@ -510,12 +634,15 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
             float dtg = (float)(dt * h->aI[i][i]);
             {
                 Launch l(h, st, KID_NEWTON_STAGE);
-                k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i], h->scr);
+                if (h->newton_variant == 1 || h->newton_variant == 2)
+                    launch_newton_pcr(h, src, dtg, i, st);
+                else
+                    k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i], h->scr);
             }
             if ((s = check_launch(h, "k_newton_stage"))) return s;
-            if ((s = explicit_tendency(h, h->FE[i], h->U, st))) return s;
-            if ((s = dss_state(h, h->FE[i], st, KID_DSS_TENDENCY))) return s;
+            if ((s = explicit_tendency(h, h->FE[i], h->U, st, true))) return s;
         }
+        // u_{n+1} = u + dt sum b (F_E + F_I), then dss! and the filter
         FinalSrc fs{};
         fs.dt = (float)dt;
         for (int j = 0; j < 4; ++j)
@@ -531,9 +658,14 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
             }
         {
             Launch l(h, st, KID_FINAL);
-            k_final<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, Y, fs);
+            k_combine<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(g, Y, h->U, fs);
         }
-        if ((s = check_launch(h, "k_final"))) return s;
+        if ((s = check_launch(h, "k_combine"))) return s;
+        if (g.nelem > 0 && g.ndss > 0) {
+            if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
+        } else {
+            if ((s = filter_columns(h, h->U, Y, st))) return s;
+        }
     }
     return CA_OK;
 }
@@ -618,6 +750,37 @@ ca_status ca_profile_enable(ca_handle *h, int32_t enable) {
 }
 
 int32_t ca_profile_count(void) { return KID_COUNT; }
+
+ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value) {
+    if (!h) return CA_ERR_ARG;
+    if (knob == CA_TUNE_NEWTON) {
+        if (value < 0 || value > 2) return fail(h, CA_ERR_ARG, "CA_TUNE_NEWTON must be 0, 1 or 2");
+        if (value > 0 && h->g.nlev + 1 > 64)
+            return fail(h, CA_ERR_ARG, "wavefront-per-column Newton stage needs nlev + 1 <= 64");
+        h->newton_variant = value;
+        return CA_OK;
+    }
+    if (knob == CA_TUNE_EXPLICIT_CHUNK) {
+        if (value < 0 || value > h->g.nlev) return fail(h, CA_ERR_ARG, "chunk must be in [0, nlev]");
+        h->explicit_chunk = value;
+        return CA_OK;
+    }
+    if (knob == CA_TUNE_DSS_CHUNK) {
+        if (value < 1 || value > h->g.nlev) return fail(h, CA_ERR_ARG, "DSS chunk must be in [1, nlev]");
+        h->dss_chunk = value;
+        return CA_OK;
+    }
+    return fail(h, CA_ERR_ARG, "unknown tuning knob");
+}
+
+ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value) {
+    if (!h || !value) return CA_ERR_ARG;
+    if (knob == CA_TUNE_NEWTON) *value = h->newton_variant;
+    else if (knob == CA_TUNE_EXPLICIT_CHUNK) *value = h->explicit_chunk;
+    else if (knob == CA_TUNE_DSS_CHUNK) *value = h->dss_chunk;
+    else return CA_ERR_ARG;
+    return CA_OK;
+}
 
 ca_status ca_profile_get(ca_handle *h, int32_t id, const char **name, int64_t *launches, double *total_ms) {
     if (!h || id < 0 || id >= KID_COUNT) return CA_ERR_ARG;

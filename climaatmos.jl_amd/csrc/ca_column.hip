@@ -548,75 +548,54 @@ __global__ void __launch_bounds__(128) k_newton_stage(DevGeom g, DevNum n, Stage
     }
 }
 
-// Final combination u <- u + dt sum_j b_j (F_E,j + F_I,j), fused with the
-// end-of-step dss! (integrator.jl:217-223): a shared node's new value is the
-// J-weighted average over its copies, written identically to every copy, so
-// duplicated element-edge columns stay bitwise equal from step to step.
-// Boundary u_3 = 0 (impenetrability filter of the end-of-step cache!).
-// Thread per (column, level y): y < nlev handles the four centre components
-// of level y, every y the u_3 face y.  Only the first copy of a shared node
-// (the group leader) works for the group.
-__device__ __forceinline__ float comb_center(const FinalSrc &s, const float *u, int comp, int64_t q, int64_t nC) {
-    float acc = 0.0f;
-    for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][comp * nC + q], acc);
-    if (comp == 0 || comp == 3) {
-        int ci = comp == 0 ? 0 : 1;
-        for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][ci * nC + q], acc);
-    }
-    return u[comp * nC + q] + s.dt * acc;
-}
-__device__ __forceinline__ float comb_face(const FinalSrc &s, const float *u, int64_t q, int64_t nC) {
-    float acc = 0.0f;
-    for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][4 * nC + q], acc);
-    for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][2 * nC + q], acc);
-    return u[4 * nC + q] + s.dt * acc;
-}
-
-__global__ void __launch_bounds__(256) k_final(DevGeom g, float *u, FinalSrc s) {
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (col >= g.ncol) return;
-    const int64_t nC = g.nC, nc = g.ncol;
-    const int grp = g.col_group ? g.col_group[col] : -1;
-    const bool bnd_face = (y == 0 || y == g.nlev);
-    if (grp < 0) {
-        int64_t q = (int64_t)y * nc + col;
-        float v[4];
-        if (y < g.nlev)
-            for (int comp = 0; comp < 4; ++comp) v[comp] = comb_center(s, u, comp, q, nC);
-        float w = bnd_face ? 0.0f : comb_face(s, u, q, nC);
-        if (y < g.nlev)
-            for (int comp = 0; comp < 4; ++comp) u[comp * nC + q] = v[comp];
-        u[4 * nC + q] = w;
+// Final combination out = u + dt sum_j b_j (F_E,j + F_I,j) (the DSS and the
+// impenetrability filter of the end of the step follow in k_dss_gather).
+__global__ void __launch_bounds__(256) k_combine(DevGeom g, const float *u, float *out, FinalSrc s) {
+    // four consecutive values per thread (every plane size is a multiple of 4
+    // when ncol is; otherwise the scalar tail path is taken)
+    const int64_t S = 4 * g.nC + g.nF;
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= S) return;
+    const bool vec = (g.ncol & 3) == 0 && i0 + 3 < S;
+    const int comp = i0 < 4 * g.nC ? (int)(i0 / g.nC) : 4;
+    const int64_t r = i0 - (int64_t)comp * g.nC;
+    const bool has_fi = comp == 0 || comp == 3 || comp == 4;
+    const int ci = comp == 0 ? 0 : (comp == 3 ? 1 : 2);
+    if (vec) {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int t = 0; t < s.nfe; ++t) {
+            const float4 v = *reinterpret_cast<const float4 *>(s.fe[t] + i0);
+            acc.x = fmaf(s.be[t], v.x, acc.x);
+            acc.y = fmaf(s.be[t], v.y, acc.y);
+            acc.z = fmaf(s.be[t], v.z, acc.z);
+            acc.w = fmaf(s.be[t], v.w, acc.w);
+        }
+        if (has_fi)
+            for (int t = 0; t < s.nfi; ++t) {
+                const float4 v = *reinterpret_cast<const float4 *>(s.fi[t] + ci * g.nC + r);
+                acc.x = fmaf(s.bi[t], v.x, acc.x);
+                acc.y = fmaf(s.bi[t], v.y, acc.y);
+                acc.z = fmaf(s.bi[t], v.z, acc.z);
+                acc.w = fmaf(s.bi[t], v.w, acc.w);
+            }
+        const float4 u4 = *reinterpret_cast<const float4 *>(u + i0);
+        float4 o;
+        o.x = u4.x + s.dt * acc.x;
+        o.y = u4.y + s.dt * acc.y;
+        o.z = u4.z + s.dt * acc.z;
+        o.w = u4.w + s.dt * acc.w;
+        *reinterpret_cast<float4 *>(out + i0) = o;
         return;
     }
-    const int b = g.dptr[grp], e = g.dptr[grp + 1];
-    if (g.dcol[b] != col) return;  // not the leader
-    const float *B = g.basis;
-    float sr = 0.0f, se = 0.0f, sw = 0.0f, X0 = 0.0f, X1 = 0.0f, X2 = 0.0f;
-    for (int t = b; t < e; ++t) {
-        const int c = g.dcol[t];
-        const float wt = g.dw[t];
-        const int64_t q = (int64_t)y * nc + c;
-        if (y < g.nlev) {
-            sr = fmaf(wt, comb_center(s, u, 0, q, nC), sr);
-            se = fmaf(wt, comb_center(s, u, 3, q, nC), se);
-            float a = wt * comb_center(s, u, 1, q, nC), bb = wt * comb_center(s, u, 2, q, nC);
-            X0 += a * B[0 * nc + c] + bb * B[3 * nc + c];
-            X1 += a * B[1 * nc + c] + bb * B[4 * nc + c];
-            X2 += a * B[2 * nc + c] + bb * B[5 * nc + c];
+    for (int64_t idx = i0; idx < i0 + 4 && idx < S; ++idx) {
+        const int cp = idx < 4 * g.nC ? (int)(idx / g.nC) : 4;
+        const int64_t rr = idx - (int64_t)cp * g.nC;
+        float acc = 0.0f;
+        for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][idx], acc);
+        if (cp == 0 || cp == 3 || cp == 4) {
+            const int cj = cp == 0 ? 0 : (cp == 3 ? 1 : 2);
+            for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][cj * g.nC + rr], acc);
         }
-        if (!bnd_face) sw = fmaf(wt, comb_face(s, u, q, nC), sw);
-    }
-    for (int t = b; t < e; ++t) {
-        const int c = g.dcol[t];
-        const int64_t q = (int64_t)y * nc + c;
-        if (y < g.nlev) {
-            u[q] = sr;
-            u[3 * nC + q] = se;
-            u[nC + q] = X0 * B[6 * nc + c] + X1 * B[7 * nc + c] + X2 * B[8 * nc + c];
-            u[2 * nC + q] = X0 * B[9 * nc + c] + X1 * B[10 * nc + c] + X2 * B[11 * nc + c];
-        }
-        u[4 * nC + q] = bnd_face ? 0.0f : sw;
+        out[idx] = u[idx] + s.dt * acc;
     }
 }

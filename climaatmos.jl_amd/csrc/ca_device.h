@@ -30,6 +30,12 @@ struct DevGeom {
     const double *dPhif_d;               // dPhif in float64
     const double *Phic_d;                // g zc in float64
     const int *col_group;                // shared-node group of a column (-1: interior node)
+    // per-column gather table of the DSS, slot-major planes of ncol: slot 0 is
+    // the column itself; gidx[4] member columns, gw[4] weights, grank packs the
+    // canonical (group-order) rank of each slot in 2 bits, gM[16] the 2x2
+    // covariant transform w_s (a_i(own) . a^j(slot)) of each slot, gm = count
+    const int *gidx, *grank, *gm;
+    const float *gw, *gM;
     const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
     // DSS groups
     int ndss;
@@ -88,46 +94,69 @@ __device__ __forceinline__ void ca_cov(const DevGeom &g, int col, float s2, floa
 }
 
 // ------------------------------------------------- element-local derivatives
-// The 16 nodes of an element occupy 16 consecutive lanes of a wavefront
-// (lane = 16 e + 4 j + i).  Values of neighbouring nodes are exchanged with
-// cross-lane permutes; D stays in constant memory.
+// The 16 nodes of an element occupy one 16-lane DPP row of a wavefront
+// (lane = 16 e + 4 j + i).  Along the first reference coordinate the four
+// values of a node's line are quad broadcasts (DPP quad_perm); along the
+// second they are row rotations by 4, 8, 12 lanes (DPP row_ror), each node
+// weighting a rotation with its own coefficient of D.  Both are plain VALU
+// moves: no LDS traffic, no permute latency.
 struct Nod {
     int base, i, j;  // lane of node (0, 0) of the element; node indices
+    float c1[4];     // D[i][m]            (strong, first coordinate)
+    float c1t[4];    // D[m][i]            (weak / transpose)
+    float c2[4];     // D[j][(j - k) & 3]  (strong, second coordinate, rotation 4k)
+    float c2t[4];    // D[(j - k) & 3][j]
 };
 __device__ __forceinline__ Nod ca_node(int lane) {
     Nod q;
     q.base = lane & ~15;
     q.i = lane & 3;
     q.j = (lane >> 2) & 3;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        q.c1[m] = c_D[q.i][m];
+        q.c1t[m] = c_D[m][q.i];
+        q.c2[m] = c_D[q.j][(q.j - m) & 3];
+        q.c2t[m] = c_D[(q.j - m) & 3][q.j];
+    }
     return q;
 }
-__device__ __forceinline__ float ca_sh(float v, int src) { return __shfl(v, src, 64); }
+template <int CTRL>
+__device__ __forceinline__ float ca_dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// value at node (0, 0) of the element (DPP row_newbcast:0)
+__device__ __forceinline__ float ca_elem0(float v) { return ca_dpp<0x150>(v); }
+// quad broadcast of element m: quad_perm(m, m, m, m)
+#define CA_QB(m) ((m) | ((m) << 2) | ((m) << 4) | ((m) << 6))
+// row rotate right by r lanes (within the 16-lane row)
+#define CA_ROR(r) (0x120 + (r))
 // sum_m D[i][m] f(m, j)
 __device__ __forceinline__ float ca_d1(const Nod &q, float f) {
-    float r = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) r = fmaf(c_D[q.i][m], ca_sh(f, q.base + 4 * q.j + m), r);
-    return r;
-}
-// sum_m D[j][m] f(i, m)
-__device__ __forceinline__ float ca_d2(const Nod &q, float f) {
-    float r = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) r = fmaf(c_D[q.j][m], ca_sh(f, q.base + 4 * m + q.i), r);
-    return r;
+    float r = q.c1[0] * ca_dpp<CA_QB(0)>(f);
+    r = fmaf(q.c1[1], ca_dpp<CA_QB(1)>(f), r);
+    r = fmaf(q.c1[2], ca_dpp<CA_QB(2)>(f), r);
+    return fmaf(q.c1[3], ca_dpp<CA_QB(3)>(f), r);
 }
 // sum_m D[m][i] g(m, j)   (transpose: weak forms)
 __device__ __forceinline__ float ca_d1t(const Nod &q, float f) {
-    float r = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) r = fmaf(c_D[m][q.i], ca_sh(f, q.base + 4 * q.j + m), r);
-    return r;
+    float r = q.c1t[0] * ca_dpp<CA_QB(0)>(f);
+    r = fmaf(q.c1t[1], ca_dpp<CA_QB(1)>(f), r);
+    r = fmaf(q.c1t[2], ca_dpp<CA_QB(2)>(f), r);
+    return fmaf(q.c1t[3], ca_dpp<CA_QB(3)>(f), r);
+}
+// sum_m D[j][m] f(i, m): rotation by 4k brings f(i, (j - k) & 3)
+__device__ __forceinline__ float ca_d2(const Nod &q, float f) {
+    float r = q.c2[0] * f;
+    r = fmaf(q.c2[1], ca_dpp<CA_ROR(4)>(f), r);
+    r = fmaf(q.c2[2], ca_dpp<CA_ROR(8)>(f), r);
+    return fmaf(q.c2[3], ca_dpp<CA_ROR(12)>(f), r);
 }
 __device__ __forceinline__ float ca_d2t(const Nod &q, float f) {
-    float r = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) r = fmaf(c_D[m][q.j], ca_sh(f, q.base + 4 * m + q.i), r);
-    return r;
+    float r = q.c2t[0] * f;
+    r = fmaf(q.c2t[1], ca_dpp<CA_ROR(4)>(f), r);
+    r = fmaf(q.c2t[2], ca_dpp<CA_ROR(8)>(f), r);
+    return fmaf(q.c2t[3], ca_dpp<CA_ROR(12)>(f), r);
 }
 
 // Spectral-element operators at one level (docs/src/discretization.md:72-135).

@@ -45,12 +45,17 @@ __global__ void k_update_jacobian(DevGeom g, DevNum n, const float *Y, float dtg
 __global__ void k_ldiv(DevGeom g, Bands B, float *dY, const float *R, float *scr);
 __global__ void k_correct(DevGeom g, DevNum n, float *Yt, const float *Y);
 __global__ void k_newton_stage(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI, float *scr);
-__global__ void k_final(DevGeom g, float *u, FinalSrc s);
+template <int TC, int NF, int NI>
+__global__ void k_newton_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI);
 
 // explicit tendency (ca_spectral.hip)
 __global__ void k_laplacians(DevGeom g, DevNum n, const float *Y, float *L);
-__global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt);
+template <bool SPH, bool HYPER, bool VDIFF>
+__global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk);
 
 // DSS (ca_dss.hip): weighted average over shared nodes, level range of planes
 __global__ void k_dss_scalar(DevGeom g, float *f, int nplanes, int nlev_plane, int64_t plane_stride);
 __global__ void k_dss_vector(DevGeom g, float *u1, float *u2, int nlev_plane);
+__global__ void k_dss_gather(DevGeom g, const float *src, float *dst, int ncp, int vec, int face, int filter,
+                             int chunk);
+__global__ void k_combine(DevGeom g, const float *u, float *out, FinalSrc s);

@@ -1,0 +1,256 @@
+// ca_newton_pcr.hip -- the fused implicit ARS343 stage with one wavefront per
+// column and one lane per level (nlev + 1 <= 64 faces).
+//
+// Same algorithm as k_newton_stage (ca_column.hip): stage-state assembly,
+// impenetrability filter, one Newton iteration with the manual Jacobian of
+// manual_sparse_jacobian.jl:713-870, Schur complement on u_3
+// (manual_sparse_jacobian.jl:579-584), post-Newton upwind correction
+// (implicit_tendency.jl:322-339), and F_I = (U - U~) / dtg.  The tridiagonal
+// Schur system is solved by parallel cyclic reduction across the 64 lanes.
+//
+// A workgroup owns a tile of TC adjacent columns: the tile is assembled from
+// the sources with coalesced row loads into LDS, each wavefront then solves
+// its columns lane-per-level from LDS, and the results leave through LDS
+// with coalesced row stores.
+#include "ca_kernels.h"
+
+namespace pcr {
+
+template <class T>
+__device__ __forceinline__ T up(T v, int d) { return __shfl_up(v, d, 64); }
+template <class T>
+__device__ __forceinline__ T dn(T v, int d) { return __shfl_down(v, d, 64); }
+
+}  // namespace pcr
+
+// NF / NI: compile-time numbers of full-state and implicit-tendency sources
+// (the three ARS343 implicit stages have (2, 0), (3, 1), (4, 2)); -1 reads
+// the counts from src.
+template <int NF, int NI>
+__device__ __forceinline__ void assemble(const StageSrc &src, int64_t nC, int64_t q, bool okc, bool cen,
+                                         bool face, float out[5]) {
+    const int nf = NF >= 0 ? NF : src.nfull;
+    const int ni = NI >= 0 ? NI : src.nfi;
+#pragma unroll
+    for (int comp = 0; comp < 4; ++comp) {
+        float s = 0.0f;
+        if (okc && cen) {
+#pragma unroll
+            for (int t = 0; t < (NF >= 0 ? NF : CA_MAX_SRC); ++t)
+                if (t < nf) s = fmaf(src.cf[t], src.full[t][comp * nC + q], s);
+            if (comp == 0 || comp == 3) {
+                const int ci = comp == 0 ? 0 : 1;
+#pragma unroll
+                for (int t = 0; t < (NI >= 0 ? NI : CA_MAX_SRC); ++t)
+                    if (t < ni) s = fmaf(src.ci[t], src.fi[t][ci * nC + q], s);
+            }
+        }
+        out[comp] = s;
+    }
+    float s = 0.0f;
+    if (okc && face) {
+#pragma unroll
+        for (int t = 0; t < (NF >= 0 ? NF : CA_MAX_SRC); ++t)
+            if (t < nf) s = fmaf(src.cf[t], src.full[t][4 * nC + q], s);
+#pragma unroll
+        for (int t = 0; t < (NI >= 0 ? NI : CA_MAX_SRC); ++t)
+            if (t < ni) s = fmaf(src.ci[t], src.fi[t][2 * nC + q], s);
+    }
+    out[4] = s;
+}
+
+template <int TC, int NF, int NI>
+__global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U,
+                                                    float *FI) {
+    constexpr int R = TC + 1;  // padded LDS row (conflict-free column reads)
+    __shared__ float tU[5][64][R];
+    __shared__ float tF[3][64][R];
+    const int nl = g.nlev;
+    const int64_t nC = g.nC, nc = g.ncol;
+    const int col0 = blockIdx.x * TC;
+    const int tid = threadIdx.x;
+
+    // ---- phase 1: assemble U~ (coalesced rows of TC columns)
+#pragma unroll 2
+    for (int idx = tid; idx < 64 * TC; idx += 256) {
+        const int r = idx / TC, cc = idx - r * TC;
+        const int col = col0 + cc;
+        float v[5];
+        assemble<NF, NI>(src, nC, (int64_t)r * nc + col, col < nc, r < nl, r >= 1 && r <= nl - 1, v);
+#pragma unroll
+        for (int comp = 0; comp < 5; ++comp) tU[comp][r][cc] = v[comp];
+    }
+    __syncthreads();
+
+    // ---- phase 2: one column per wavefront, lane k = centre k and face k
+    const int k = tid & 63;
+    const int wid = tid >> 6;
+    const bool hasC = k < nl;
+    const bool intF = k >= 1 && k <= nl - 1;
+    const bool isF = k <= nl;
+    const int kc = hasC ? k : nl - 1;   // clamped centre level for padding lanes
+    const int kf = isF ? k : nl;        // clamped face level
+    const float Jvf = g.Jvf[kf];
+    const float g33 = 1.0f / (Jvf * Jvf);
+    const float Jvf1 = g.Jvf[kc + 1];
+    const float g33p = 1.0f / (Jvf1 * Jvf1);
+    for (int cc = wid; cc < TC; cc += 4) {
+        const int col = col0 + cc;
+        if (col >= nc) break;  // uniform across the wavefront
+        const float rho = tU[0][kc][cc], u1 = tU[1][kc][cc], u2 = tU[2][kc][cc], rhoe = tU[3][kc][cc];
+        const float u3 = intF ? tU[4][k][cc] : 0.0f;
+        const float u3p = pcr::dn(u3, 1);  // face k + 1
+        const float u3m = pcr::up(u3, 1);  // face k - 1
+        const CV c = cv_make(g, n, kc, col, rho, u1, u2, rhoe, u3, hasC ? u3p : 0.0f);
+        const float rJ = c.rho * c.J;
+        // centre row k: A(rho,u3), A(rhoe,u3) on faces k, k+1 and R = dtg T_imp
+        const float rJm = pcr::up(rJ, 1), hm = pcr::up(c.h, 1);
+        const float rJp = pcr::dn(rJ, 1), hp = pcr::dn(c.h, 1);
+        float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f, Rr = 0.0f, Re = 0.0f;
+        if (hasC) {
+            float Fm = 0.0f, Fp = 0.0f, Hm = 0.0f, Hp = 0.0f;
+            if (k >= 1) {
+                const float rJf = 0.5f * (rJm + rJ);
+                a0 = dtg * (rJf / c.J) * g33;
+                e0 = a0 * (0.5f * (hm + c.h));
+                Fm = rJf * (u3 * g33);
+                Hm = Fm * (0.5f * (hm + c.h));
+            }
+            if (k + 1 <= nl - 1) {
+                const float rJf = 0.5f * (rJ + rJp);
+                a1 = dtg * (-rJf / c.J) * g33p;
+                e1 = a1 * (0.5f * (c.h + hp));
+                Fp = rJf * (u3p * g33p);
+                Hp = Fp * (0.5f * (c.h + hp));
+            }
+            Rr = dtg * (-(Fp - Fm) / c.J);
+            Re = dtg * (-(Hp - Hm) / c.J);
+        }
+        // face row k: needs centre k - 1 (lane k - 1) and centre k (own lane)
+        CV a;
+        a.rho = pcr::up(c.rho, 1);
+        a.K = pcr::up(c.K, 1);
+        a.Td = pcr::up(c.Td, 1);
+        a.pd = pcr::up(c.pd, 1);
+        a.Pid = pcr::up(c.Pid, 1);
+        a.T = (float)a.Td;
+        a.p = (float)a.pd;
+        a.Pi = (float)a.Pid;
+        a.ct1 = pcr::up(c.ct1, 1);
+        a.ct2 = pcr::up(c.ct2, 1);
+        const float pa0 = pcr::up(a0, 1), pa1 = pcr::up(a1, 1), pe0 = pcr::up(e0, 1), pe1 = pcr::up(e1, 1);
+        const float pRr = pcr::up(Rr, 1), pRe = pcr::up(Re, 1);
+        float L = 0.0f, D = 1.0f, Up = 0.0f, rhs = 0.0f;
+        if (intF) {
+            const FaceRow fr = jac_face_row(g, n, k, col, a, c, u3m, u3, u3p, dtg);
+            const float Ru3 = dtg * (pgf_u3(g, n, k, a, c) - beta_w(g, n, k, col) * u3);
+            L = fr.lo + fr.r0 * pa0 + fr.e0 * pe0;
+            D = fr.d + fr.r0 * pa1 + fr.e0 * pe1 + fr.r1 * a0 + fr.e1 * e0;
+            Up = fr.up + fr.r1 * a1 + fr.e1 * e1;
+            rhs = Ru3 + fr.r0 * pRr + fr.r1 * Rr + fr.e0 * pRe + fr.e1 * Re;
+        } else if (isF) {
+            D = -dtg * beta_w(g, n, k, col) - 1.0f;
+        }
+        // parallel cyclic reduction on the 64 face equations, in float64: the
+        // acoustic Schur system is stiff (dtg c / dz >> 1) and PCR's
+        // elimination order loses more in float32 than a Thomas sweep
+        double Ld = L, Dd = D, Ud = Up, rd = rhs;
+#pragma unroll
+        for (int s = 1; s < 64; s <<= 1) {
+            const bool lo_ok = k >= s, hi_ok = k + s <= 63;
+            double La = pcr::up(Ld, s), Da = pcr::up(Dd, s), Ua = pcr::up(Ud, s), ra = pcr::up(rd, s);
+            double Lb = pcr::dn(Ld, s), Db = pcr::dn(Dd, s), Ub = pcr::dn(Ud, s), rb = pcr::dn(rd, s);
+            if (!lo_ok) {
+                La = 0.0;
+                Da = 1.0;
+                Ua = 0.0;
+                ra = 0.0;
+            }
+            if (!hi_ok) {
+                Lb = 0.0;
+                Db = 1.0;
+                Ub = 0.0;
+                rb = 0.0;
+            }
+            const double al = -Ld / Da, be = -Ud / Db;
+            const double nL = al * La, nU = be * Ub;
+            const double nD = Dd + al * Ua + be * Lb;
+            const double nr = rd + al * ra + be * rb;
+            Ld = nL;
+            Ud = nU;
+            Dd = nD;
+            rd = nr;
+        }
+        const float x = (float)(rd / Dd);  // du_3 at face k
+        const float x1 = pcr::dn(x, 1);   // du_3 at face k + 1
+        // Newton update
+        const float dr = a0 * x + a1 * x1 - Rr;
+        const float de = e0 * x + e1 * x1 - Re;
+        const float rho_n = rho - dr;
+        float rhoe_n = rhoe - de;
+        const float u3_n = intF ? u3 - x : 0.0f;
+        float fi_e = -de / dtg;
+        // post-Newton upwind correction at the new state
+        if (n.upw != 0) {
+            const float u3_np = pcr::dn(u3_n, 1);
+            const CV cn = cv_make(g, n, kc, col, rho_n, u1, u2, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
+            const float href = __shfl(cn.h, 0, 64);
+            const float hr = cn.h - href;
+            const float rJn = cn.rho * cn.J;
+            const float h2 = pcr::up(hr, 2), h1 = pcr::up(hr, 1), h4 = pcr::dn(hr, 1);
+            const float rJn1 = pcr::up(rJn, 1);
+            float dF = 0.0f;
+            if (intF) {
+                const float v = u3_n * g33;
+                const float rJf = 0.5f * (rJn1 + rJn);
+                const float upw = upwind_product(n.upw, k, nl, v, n.dt, h2, h1, hr, h4);
+                const float cen = upwind_product(0, k, nl, v, n.dt, 0.0f, h1, hr, 0.0f);
+                dF = rJf * upw - rJf * cen;
+            }
+            const float dF1 = pcr::dn(dF, 1);
+            const float corr = hasC ? -(dF1 - dF) / cn.J : 0.0f;
+            rhoe_n += dtg * corr;
+            fi_e += corr;
+        }
+        if (hasC) {
+            tU[0][k][cc] = rho_n;
+            tU[3][k][cc] = rhoe_n;
+            tF[0][k][cc] = -dr / dtg;
+            tF[1][k][cc] = fi_e;
+        }
+        if (isF) {
+            tU[4][k][cc] = u3_n;
+            tF[2][k][cc] = intF ? -x / dtg : 0.0f;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: coalesced stores of U and F_I
+    for (int idx = tid; idx < 64 * TC; idx += 256) {
+        const int r = idx / TC, cc = idx - r * TC;
+        const int col = col0 + cc;
+        if (col >= nc) continue;
+        const int64_t q = (int64_t)r * nc + col;
+        if (r < nl) {
+#pragma unroll
+            for (int comp = 0; comp < 4; ++comp) U[comp * nC + q] = tU[comp][r][cc];
+            FI[q] = tF[0][r][cc];
+            FI[nC + q] = tF[1][r][cc];
+        }
+        if (r <= nl) {
+            U[4 * nC + q] = tU[4][r][cc];
+            FI[2 * nC + q] = tF[2][r][cc];
+        }
+    }
+}
+
+#define CA_INST_PCR(TC, NF, NI) \
+    template __global__ void k_newton_pcr<TC, NF, NI>(DevGeom, DevNum, StageSrc, float, float *, float *);
+CA_INST_PCR(16, 2, 0)
+CA_INST_PCR(16, 3, 1)
+CA_INST_PCR(16, 4, 2)
+CA_INST_PCR(16, -1, -1)
+CA_INST_PCR(32, 2, 0)
+CA_INST_PCR(32, 3, 1)
+CA_INST_PCR(32, 4, 2)
+CA_INST_PCR(32, -1, -1)

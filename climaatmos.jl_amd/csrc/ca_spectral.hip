@@ -11,8 +11,8 @@
 //   src/parameterized_tendencies/sponge/rayleigh_sponge.jl:39-44
 //
 // Mapping: one lane per horizontal node, the 16 nodes of an element in 16
-// consecutive lanes; horizontal derivatives are cross-lane permutes, vertical
-// stencils are a register window carried up the column.
+// consecutive lanes (one DPP row); horizontal derivatives are DPP lane moves,
+// vertical stencils are a register window carried up the column.
 #include "ca_kernels.h"
 
 __device__ __forceinline__ float u3f_at(const DevGeom &g, const float *u3, int f, int col) {
@@ -43,9 +43,10 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
     ca_contra(g, col, is2, u1, u2, ct1, ct2);
     float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]), gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
     float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
-    float e_tot = rhoe / rho;
-    float T = ca_temperature(n, e_tot - K - g.Phic[k]);
-    float p = rho * n.R_d * T;
+    const double e_tot_d = (double)rhoe / (double)rho;
+    const double Td = fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) / n.dcv_d, n.dTmin);
+    float T = (float)Td;
+    float p = (float)((double)rho * n.dR_d * Td);
     float Pi = ca_exner(n, p);
     float cu3 = 0.5f * (u3a + u3b);
     // vector Laplacian of c_u
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
     L[nC + idx] = gd2 - wc2;
     L[2 * nC + idx] = 0.0f - wc3;
     // scalar Laplacian of the dry static energy deviation
-    float p0 = ca_sh(p, q.base), Pi0 = ca_sh(Pi, q.base);
+    float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi);
     float psi = n.cp_d * (T - ca_Tref(n, Pi)) - ca_dPhiref(n, p0, Pi0, p);
     float q1, q2;
     op_grad(q, psi, q1, q2);
@@ -79,33 +80,45 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
 
 // --------------------------------------------------------- explicit kernel
 struct ColState {
-    float rhoJ, K, ct1, ct2, u1, u2, rho;
+    float rhoJ, Kh, Kv, ct1, ct2, u1, u2, rho;  // K = Kh + Kv (horizontal, vertical parts)
     float l4;                  // C3 of nabla^4 u (hyperdiffusion)
     float dr, d1, d2, de;      // partial centre tendencies
     float sd;                  // dry static energy (vertical diffusion)
     float uc0, uc1, uc2;       // velocity in the diffusion basis
 };
 
-__global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt) {
+// Work item = (node, chunk of levels): chunk c owns centres and faces
+// [c * chunk, (c + 1) * chunk); it recomputes the centre below its first
+// level (for the face stencils) and the centre above its last level (for the
+// face term of its top centre).  chunk = 0 means the whole column.
+// SPH / HYPER / VDIFF select the sphere (horizontal operators), the
+// hyperdiffusion and the vertical diffusion at compile time.
+template <bool SPH, bool HYPER, bool VDIFF>
+__global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt,
+                                                  int chunk) {
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.ncol) return;
     const Nod q = ca_node(threadIdx.x & 63);
     const int nl = g.nlev;
     const int64_t nC = g.nC, nc = g.ncol;
     const float *u3 = Y + 4 * nC;
-    const bool sph = n.sphere != 0;
+    constexpr bool sph = SPH;
     const float W = c_W[q.j * 4 + q.i];
     const float Jh = g.Jh[col];
     const float f3n = g.f3n[col], f12a = g.f12a[col], f12b = g.f12b[col];
     float Kh = 0.0f;
-    if (n.vdiff) Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
+    if (VDIFF) Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
     ColState prev, cur;
-    prev.rhoJ = prev.K = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
+    prev.rhoJ = prev.Kh = prev.Kv = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
     prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = 0.0f;
     float Gp1 = 0.0f, Gp2 = 0.0f;      // G at the face below the current centre
     float Fs_prev = 0.0f, Fu0_prev = 0.0f, Fu1_prev = 0.0f, Fu2_prev = 0.0f;  // diffusive fluxes
-    float u3a = 0.0f;                  // u_3 at face k
-    for (int k = 0; k <= nl; ++k) {
+    const int k0 = chunk > 0 ? (int)blockIdx.y * chunk : 0;
+    const int k1 = chunk > 0 ? min(k0 + chunk, nl) : nl;
+    const int kbeg = k0 > 0 ? k0 - 1 : 0;
+    const int kend = k1;  // inclusive: face k1 closes the top owned centre
+    float u3a = u3f_at(g, u3, kbeg, col);  // u_3 at face k
+    for (int k = kbeg; k <= kend; ++k) {
         float u3b = (k < nl) ? u3f_at(g, u3, k + 1, col) : 0.0f;
         if (k < nl) {
             const int64_t idx = (int64_t)k * nc + col;
@@ -115,11 +128,16 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             float ct1, ct2;
             ca_contra(g, col, is2, u1, u2, ct1, ct2);
             float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]), gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
-            float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
-            float e_tot = rhoe / rho;
-            float T = ca_temperature(n, e_tot - K - g.Phic[k]);
-            float h = e_tot + n.R_d * T;
-            float p = rho * n.R_d * T;
+            const float Khor = 0.5f * (u1 * ct1 + u2 * ct2);
+            const float Kver = 0.25f * (u3a * ga * u3a + u3b * gb * u3b);
+            float K = Khor + Kver;
+            // temperature from e_int in float64 (e_tot and Phi are large)
+            const double e_tot_d = (double)rhoe / (double)rho;
+            const double Td =
+                fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) / n.dcv_d, n.dTmin);
+            float T = (float)Td;
+            float h = (float)(e_tot_d + n.dR_d * Td);
+            float p = (float)((double)rho * n.dR_d * Td);
             float Pi = ca_exner(n, p);
             float dr = 0.0f, d1 = 0.0f, d2 = 0.0f, de = 0.0f, l4 = 0.0f;
             if (sph) {
@@ -129,14 +147,14 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 dr -= 0.5f * wd + 0.5f * wd;  // split_div(rho u, 1)
                 // split_div(rho u, h) = 1/2 wdiv(rho u h) + 1/2 (h wdiv(rho u) + rho u . grad h),
                 // with h taken relative to the element's first node (exact: linear in constants)
-                float h0 = ca_sh(h, q.base);
+                float h0 = ca_elem0(h);
                 float hh = h - h0;
                 float wdh = op_wdiv(q, W, J, m1 * hh, m2 * hh);
                 float gh1, gh2;
                 op_grad(q, hh, gh1, gh2);
                 de -= 0.5f * wdh + 0.5f * (hh * wd + (m1 * gh1 + m2 * gh2)) + h0 * wd;
                 // u_h -= C12(grad(K + Phi - Phi_r) + cp_d (th' grad Pi + grad(th' Pi) - Pi grad th') / 2)
-                float p0 = ca_sh(p, q.base), Pi0 = ca_sh(Pi, q.base);
+                float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi);
                 float A = K - ca_dPhiref(n, p0, Pi0, p);
                 float dPi = ca_dExner(n, p0, Pi0, p);
                 float thPi = T - ca_Tref(n, Pi);
@@ -149,7 +167,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 d1 -= g1 + n.cp_d * (th * a1 + b1 - Pi * c1) * 0.5f;
                 d2 -= g2 + n.cp_d * (th * a2 + b2 - Pi * c2) * 0.5f;
                 // ---- apply_hyperdiffusion_tendency! (hyperdiffusion.jl:273-307)
-                if (n.hyper) {
+                if (HYPER) {
                     float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
                     float v1, v2;
                     ca_contra(g, col, is2, X1, X2, v1, v2);
@@ -190,7 +208,8 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 d2 += -beta * u2;
             }
             cur.rhoJ = rho * J;
-            cur.K = K;
+            cur.Kh = Khor;
+            cur.Kv = Kver;
             cur.ct1 = ct1;
             cur.ct2 = ct2;
             cur.u1 = u1;
@@ -201,7 +220,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             cur.d1 = d1;
             cur.d2 = d2;
             cur.de = de;
-            if (n.vdiff) {
+            if (VDIFF) {
                 cur.sd = n.cp_d * (T - n.T_0) + g.Phic[k];
                 if (sph) {
                     const float *B = g.basis;
@@ -235,12 +254,14 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 G1 = Jf * (A2 * b3);
                 G2 = Jf * (0.0f - A1 * b3);
                 float i1 = 0.5f * (prev.ct1 + cur.ct1), i2 = 0.5f * (prev.ct2 + cur.ct2);
-                du3 = -(Jf * (A1 * i2 - A2 * i1) + (cur.K - prev.K));
-                if (sph && n.hyper) {
+                // grad K differenced part by part: the vertical part is a small
+                // difference riding on the horizontal kinetic energy
+                du3 = -(Jf * (A1 * i2 - A2 * i1) + ((cur.Kh - prev.Kh) + (cur.Kv - prev.Kv)));
+                if (SPH && HYPER) {
                     float wA = prev.rhoJ, wB = cur.rhoJ;
                     du3 -= n.nu_v * ((0.5f * (wA * prev.l4 + wB * cur.l4)) / (0.5f * (wA + wB)));
                 }
-                if (n.vdiff) {
+                if (VDIFF) {
                     // vertical_diffusion_boundary_layer.jl:90-103 with K_h = D0 (H = Inf)
                     float rK = (0.5f * (prev.rho + cur.rho)) / (0.5f * (1.0f / Kh + 1.0f / Kh));
                     float iJv = 1.0f / g.Jvf[k];
@@ -251,16 +272,16 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     Fu2 = fac * (cur.uc2 - prev.uc2) * iJv;
                 }
             }
-            Yt[4 * nC + (int64_t)k * nc + col] = du3;
+            if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) Yt[4 * nC + (int64_t)k * nc + col] = du3;
         }
         // ---- centre k-1 is complete
-        if (k >= 1) {
+        if (k - 1 >= k0 && k - 1 < k1) {
             const int kk = k - 1;
             const int64_t idx = (int64_t)kk * nc + col;
             float d1 = prev.d1 - (0.5f * (Gp1 + G1)) / prev.rhoJ;
             float d2 = prev.d2 - (0.5f * (Gp2 + G2)) / prev.rhoJ;
             float de = prev.de;
-            if (n.vdiff) {
+            if (VDIFF) {
                 float s = g.sc[kk];
                 float Jc = Jh * s * s * g.Jvc[kk];
                 de -= (Fs - Fs_prev) / Jc;
@@ -292,3 +313,10 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
         u3a = u3b;
     }
 }
+
+template __global__ void k_explicit<true, true, false>(DevGeom, DevNum, const float *, const float *, float *, int);
+template __global__ void k_explicit<true, false, false>(DevGeom, DevNum, const float *, const float *, float *, int);
+template __global__ void k_explicit<true, true, true>(DevGeom, DevNum, const float *, const float *, float *, int);
+template __global__ void k_explicit<true, false, true>(DevGeom, DevNum, const float *, const float *, float *, int);
+template __global__ void k_explicit<false, false, false>(DevGeom, DevNum, const float *, const float *, float *, int);
+template __global__ void k_explicit<false, false, true>(DevGeom, DevNum, const float *, const float *, float *, int);

@@ -1,6 +1,7 @@
 // ca_unity.hip -- single translation unit of libcadycore (the GLL constant
 // tables live in __constant__ memory shared by all kernels).
 #include "ca_column.hip"
+#include "ca_newton_pcr.hip"
 #include "ca_spectral.hip"
 #include "ca_dss.hip"
 #include "ca_api.hip"

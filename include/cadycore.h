@@ -124,6 +124,23 @@ ca_status ca_pack_state(ca_handle *h, const float *src, const ca_layout *layouts
 ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *layouts5, float *dst,
                           void *stream);
 
+/* Launch-schedule knobs (the autotuner's search space).  Results do not
+ * depend on them beyond float32 rounding order.
+ *   CA_TUNE_NEWTON:         0 = one thread per column (Thomas sweep),
+ *                           1 = one wavefront per column, 16-column tiles (PCR),
+ *                           2 = one wavefront per column, 32-column tiles (PCR);
+ *                           1 and 2 need nlev + 1 <= 64.
+ *   CA_TUNE_EXPLICIT_CHUNK: levels per work item of the explicit-tendency
+ *                           kernel (0 = whole column).
+ *   CA_TUNE_DSS_CHUNK:      levels per work item of the DSS kernel (>= 1). */
+typedef enum ca_tune_knob {
+    CA_TUNE_NEWTON = 0,
+    CA_TUNE_EXPLICIT_CHUNK = 1,
+    CA_TUNE_DSS_CHUNK = 2
+} ca_tune_knob;
+ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value);
+ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value);
+
 /* non-finite check (NaN poisoning counterpart): *flag = 1 if Y holds NaN/Inf.
  * Synchronises the stream. */
 ca_status ca_check_finite(ca_handle *h, const float *Y, int32_t *flag, void *stream);

@@ -62,6 +62,15 @@ struct ora_ctx {
 
 #define IDX(k, col) ((int64_t)(k) * c->ncol + (col))
 
+/* The Float32 build is the CPU timing baseline and runs its column, element
+ * and node-group loops on OpenMP threads (every iteration writes only its
+ * own column / element / group); the Float64 checker is single-threaded. */
+#ifdef _OPENMP
+#define ORA_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORA_PAR_FOR
+#endif
+
 static void *zalloc(size_t n) { return calloc(n ? n : 1, 1); }
 
 int64_t ora_state_size(const ora_ctx *c) { return c->S; }
@@ -177,6 +186,7 @@ static inline real Phi_ref(const ora_ctx *c, real Pi) {
 /* One level, all elements.  Node n = 4 j + i inside an element, i along the
  * first reference coordinate.  J is the 3-D Jacobian at the level. */
 static void op_grad(const ora_ctx *c, const real *f, real *o1, real *o2) {
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e) {
         const real *fe = f + NN * e;
         for (int j = 0; j < NQ; ++j)
@@ -192,8 +202,9 @@ static void op_grad(const ora_ctx *c, const real *f, real *o1, real *o2) {
     }
 }
 static void op_div(const ora_ctx *c, const real *J, const real *v1, const real *v2, real *o) {
-    real a1[NN], a2[NN];
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e) {
+        real a1[NN], a2[NN];
         for (int q = 0; q < NN; ++q) {
             a1[q] = J[NN * e + q] * v1[NN * e + q];
             a2[q] = J[NN * e + q] * v2[NN * e + q];
@@ -207,8 +218,9 @@ static void op_div(const ora_ctx *c, const real *J, const real *v1, const real *
     }
 }
 static void op_wdiv(const ora_ctx *c, const real *J, const real *v1, const real *v2, real *o) {
-    real a1[NN], a2[NN];
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e) {
+        real a1[NN], a2[NN];
         for (int q = 0; q < NN; ++q) {
             a1[q] = c->W[q] * J[NN * e + q] * v1[NN * e + q];
             a2[q] = c->W[q] * J[NN * e + q] * v2[NN * e + q];
@@ -223,8 +235,9 @@ static void op_wdiv(const ora_ctx *c, const real *J, const real *v1, const real 
     }
 }
 static void op_wgrad(const ora_ctx *c, const real *f, real *o1, real *o2) {
-    real a[NN];
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e) {
+        real a[NN];
         for (int q = 0; q < NN; ++q) a[q] = c->W[q] * f[NN * e + q];
         for (int j = 0; j < NQ; ++j)
             for (int i = 0; i < NQ; ++i) {
@@ -240,6 +253,7 @@ static void op_wgrad(const ora_ctx *c, const real *f, real *o1, real *o2) {
     }
 }
 static void op_curl3(const ora_ctx *c, const real *J, const real *u1, const real *u2, real *o) {
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e)
         for (int j = 0; j < NQ; ++j)
             for (int i = 0; i < NQ; ++i) {
@@ -250,8 +264,9 @@ static void op_curl3(const ora_ctx *c, const real *J, const real *u1, const real
             }
 }
 static void op_wcurl3(const ora_ctx *c, const real *J, const real *u1, const real *u2, real *o) {
-    real a1[NN], a2[NN];
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e) {
+        real a1[NN], a2[NN];
         for (int q = 0; q < NN; ++q) {
             a1[q] = c->W[q] * u1[NN * e + q];
             a2[q] = c->W[q] * u2[NN * e + q];
@@ -266,6 +281,7 @@ static void op_wcurl3(const ora_ctx *c, const real *J, const real *u1, const rea
     }
 }
 static void op_curl12(const ora_ctx *c, const real *J, const real *u3, real *o1, real *o2) {
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e)
         for (int j = 0; j < NQ; ++j)
             for (int i = 0; i < NQ; ++i) {
@@ -280,8 +296,9 @@ static void op_curl12(const ora_ctx *c, const real *J, const real *u3, real *o1,
             }
 }
 static void op_wcurl12(const ora_ctx *c, const real *J, const real *u3, real *o1, real *o2) {
-    real a[NN];
+    ORA_PAR_FOR
     for (int e = 0; e < c->g.nelem; ++e) {
+        real a[NN];
         for (int q = 0; q < NN; ++q) a[q] = c->W[q] * u3[NN * e + q];
         for (int j = 0; j < NQ; ++j)
             for (int i = 0; i < NQ; ++i) {
@@ -319,6 +336,7 @@ static inline void to_cov(const ora_ctx *c, int col, real s, real a1, real a2, r
 static void dss_scalar(const ora_ctx *c, real *f, int nplanes) {
     for (int k = 0; k < nplanes; ++k) {
         real *fk = f + (int64_t)k * c->ncol;
+        ORA_PAR_FOR
         for (int n = 0; n < c->g.n_dss; ++n) {
             real s = 0;
             for (int q = c->g.dss_ptr[n]; q < c->g.dss_ptr[n + 1]; ++q)
@@ -332,6 +350,7 @@ static void dss_vector(const ora_ctx *c, real *u1, real *u2, int nplanes) {
     int64_t nc = c->ncol;
     for (int k = 0; k < nplanes; ++k) {
         real *a = u1 + (int64_t)k * nc, *b = u2 + (int64_t)k * nc;
+        ORA_PAR_FOR
         for (int n = 0; n < c->g.n_dss; ++n) {
             real X[3] = {0, 0, 0};
             for (int q = c->g.dss_ptr[n]; q < c->g.dss_ptr[n + 1]; ++q) {
@@ -369,17 +388,20 @@ void ora_dss(ora_ctx *c, ora_real *Y) { dss_state(c, (real *)Y); }
 static void set_precomputed(ora_ctx *c, real *Y) {
     int nl = c->nl, nc = c->ncol;
     real *u3 = U3(Y);
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
         u3[IDX(0, col)] = 0;
         u3[IDX(nl, col)] = 0;
     }
     const double *G = c->g.gup;
     for (int f = 0; f <= nl; ++f)
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
             c->fu3[IDX(f, col)] = g33 * u3[IDX(f, col)];
         }
     for (int k = 0; k < nl; ++k)
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             int64_t q = IDX(k, col);
             real rho = RHO(Y)[q], u1 = UH1(Y)[q], u2 = UH2(Y)[q], rhoe = RHOE(Y)[q];
@@ -405,6 +427,7 @@ static void set_precomputed(ora_ctx *c, real *Y) {
 void ora_set_precomputed(ora_ctx *c, ora_real *Y) { set_precomputed(c, (real *)Y); }
 
 void ora_get_precomputed(const ora_ctx *c, double *K, double *T, double *p, double *h) {
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) {
         K[q] = c->K[q];
         T[q] = c->T[q];
@@ -413,6 +436,7 @@ void ora_get_precomputed(const ora_ctx *c, double *K, double *T, double *p, doub
     }
 }
 void ora_get_laplacians(const ora_ctx *c, double *out) {
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) {
         out[q] = c->L1[q];
         out[c->nC + q] = c->L2[q];
@@ -465,6 +489,7 @@ static real upwind_face(const ora_ctx *c, int scheme, const real *chi, int f, in
 static void vertical_transport(const ora_ctx *c, const real *rho, const real *chi, int scheme, real *out) {
     int nl = c->nl, nc = c->ncol;
     real *F = c->t[11];
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
         F[IDX(0, col)] = 0;
         F[IDX(nl, col)] = 0;
@@ -487,15 +512,19 @@ static void implicit_tendency(ora_ctx *c, real *Yt, real *Y) {
     real *rho = RHO(Y);
     /* rho: -advdiv(I^f(rho J)/J_f u^3)  (:202) */
     real *one = c->t[10];
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) one[q] = 1;
     vertical_transport(c, rho, one, 0, c->t[9]);
     /* the central transport of chi = 1 is -advdiv(I^f(rho J)/J_f u^3 * 1) */
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) RHO(Yt)[q] += c->t[9][q];
     /* rho e_tot: central vertical transport of h_tot (:209-210) */
     vertical_transport(c, rho, c->h, 0, c->t[9]);
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) RHOE(Yt)[q] += c->t[9][q];
     /* u_3: -(grad Phi - grad Phi_r + cp_d I^f(theta_v - theta_vr) grad Pi) (:287-293) */
     real *Pi = c->t[0], *Phr = c->t[1], *thd = c->t[2];
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) {
         real P = exner(c, c->p[q]);
         Pi[q] = P;
@@ -503,6 +532,7 @@ static void implicit_tendency(ora_ctx *c, real *Yt, real *Y) {
         thd[q] = c->T[q] / P - T_ref(c, P) / P;
     }
     real cp = (real)c->n.cp_d;
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col)
         for (int f = 1; f < nl; ++f) {
             int64_t a = IDX(f - 1, col), b = IDX(f, col);
@@ -515,6 +545,7 @@ static void implicit_tendency(ora_ctx *c, real *Yt, real *Y) {
     /* Rayleigh sponge on u_3 (:295-296, rayleigh_sponge.jl:46-57) */
     if (c->n.rayleigh)
         for (int f = 0; f <= nl; ++f)
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 real beta = alpha_w_of(c, col) * rayleigh_zeta(c, c->zf[f]);
                 U3(Yt)[IDX(f, col)] += -beta * U3(Y)[IDX(f, col)];
@@ -535,6 +566,7 @@ static void update_jacobian(ora_ctx *c, real *Y, real dtg) {
     c->dtgamma = dtg;
     real *Pi = c->t[0], *thv = c->t[2], *dpdr = c->t[3];
     for (int k = 0; k < nl; ++k)
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             int64_t q = IDX(k, col);
             real P = exner(c, c->p[q]);
@@ -542,6 +574,7 @@ static void update_jacobian(ora_ctx *c, real *Y, real dtg) {
             thv[q] = c->T[q] / P;
             dpdr[q] = kap * (T0 * cp - c->K[q] - c->Phic[k]) + (Rd - kap * cv) * c->T[q];
         }
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
         /* centre rows: A(rho, u3) = dtg * adv * diag(g33), adv = -advdiv_matrix * diag(I^f(rho J)/J_f) */
         for (int k = 0; k < nl; ++k) {
@@ -634,6 +667,7 @@ void ora_update_jacobian(ora_ctx *c, ora_real *Y, double dtgamma) { update_jacob
 static void jac_ldiv(ora_ctx *c, real *dY, const real *R) {
     int nl = c->nl, nc = c->ncol;
     real *lo = c->t[4], *di = c->t[5], *up = c->t[6], *rhs = c->t[7], *cp_ = c->t[8];
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
         for (int f = 0; f <= nl; ++f) {
             int64_t q = IDX(f, col);
@@ -707,6 +741,7 @@ static void correct_implicit(ora_ctx *c, real *Yt, real *Y) {
     real *a = c->t[8], *b = c->t[9];
     vertical_transport(c, RHO(Y), c->h, c->n.upwinding, a);
     vertical_transport(c, RHO(Y), c->h, 0, b);
+    ORA_PAR_FOR
     for (int64_t q = 0; q < c->nC; ++q) RHOE(Yt)[q] = a[q] - b[q];
 }
 void ora_correct_implicit(ora_ctx *c, ora_real *Yt, ora_real *Y) { correct_implicit(c, (real *)Yt, (real *)Y); }
@@ -724,17 +759,20 @@ static void vector_laplacian_parts(ora_ctx *c, int k, const real *x1, const real
     real *cr1 = c->t[2], *cr2 = c->t[2] + nc, *cr3 = c->t[3];
     real *cc1 = c->t[4], *cc2 = c->t[4] + nc, *cc3 = c->t[5];
     real *w1 = c->t[6], *w2 = c->t[6] + nc, *w3 = c->t[7];
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) to_contra(c, col, s, x1[col], x2[col], &v1[col], &v2[col]);
     op_div(c, J, v1, v2, dv);
     op_wgrad(c, dv, gd1, gd2);
     op_curl3(c, J, x1, x2, cr3);
     op_curl12(c, J, x3, cr1, cr2);
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
         to_cov(c, col, s, cr1[col], cr2[col], &cc1[col], &cc2[col]);
         cc3[col] = Jv * Jv * cr3[col];
     }
     op_wcurl3(c, J, cc1, cc2, w3);
     op_wcurl12(c, J, cc3, w1, w2);
+    ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
         to_cov(c, col, s, w1[col], w2[col], &wc1[col], &wc2[col]);
         wc3[col] = Jv * Jv * w3[col];
@@ -759,6 +797,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
             real s = c->sc[k];
             real *m1 = c->t[0], *m2 = c->t[0] + nc, *wd = c->t[1], *wdh = c->t[1] + nc;
             real *mh1 = c->t[2], *mh2 = c->t[2] + nc, *gh1 = c->t[3], *gh2 = c->t[3] + nc;
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 real c1, c2;
                 to_contra(c, col, s, u1[o + col], u2[o + col], &c1, &c2);
@@ -769,15 +808,18 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
             }
             op_wdiv(c, J, m1, m2, wd);
             /* split_div(rho u, 1) = 1/2 wdiv(rho u) + 1/2 (wdiv(rho u) + 0) */
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) RHO(Yt)[o + col] -= wd[col] / 2 + (wd[col] + 0) / 2;
             op_wdiv(c, J, mh1, mh2, wdh);
             op_grad(c, c->h + o, gh1, gh2);
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 real sd = wdh[col] / 2 + (c->h[o + col] * wd[col] + (m1[col] * gh1[col] + m2[col] * gh2[col])) / 2;
                 RHOE(Yt)[o + col] -= sd;
             }
             /* u_h -= C12(grad(K + Phi - Phi_r) + cp_d (th' grad Pi + grad(th' Pi) - Pi grad th') / 2) */
             real *A = c->t[4], *Pi = c->t[4] + nc, *th = c->t[5], *thPi = c->t[5] + nc;
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 real P = exner(c, c->p[o + col]);
                 Pi[col] = P;
@@ -791,6 +833,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
             op_grad(c, Pi, p1, p2);
             op_grad(c, thPi, q1, q2);
             op_grad(c, th, r1, r2);
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 UH1(Yt)[o + col] -= g1[col] + cp * (th[col] * p1[col] + q1[col] - Pi[col] * r1[col]) / 2;
                 UH2(Yt)[o + col] -= g2[col] + cp * (th[col] * p2[col] + q2[col] - Pi[col] * r2[col]) / 2;
@@ -803,6 +846,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
                 int64_t o = (int64_t)k * nc;
                 real *gd1 = c->t[8], *gd2 = c->t[8] + nc, *wc1 = c->t[9], *wc2 = c->t[9] + nc, *wc3 = c->t[10];
                 vector_laplacian_parts(c, k, u1 + o, u2 + o, c->cu3 + o, gd1, gd2, wc1, wc2, wc3);
+                ORA_PAR_FOR
                 for (int col = 0; col < nc; ++col) {
                     c->L1[o + col] = gd1[col] - wc1[col];
                     c->L2[o + col] = gd2[col] - wc2[col];
@@ -810,12 +854,14 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
                 }
                 real *sd = c->t[11], *q1 = c->t[0], *q2 = c->t[0] + nc, *v1 = c->t[1], *v2 = c->t[1] + nc;
                 real cpd = (real)c->n.cp_d, T0 = (real)c->n.T_0;
+                ORA_PAR_FOR
                 for (int col = 0; col < nc; ++col) {
                     real P = exner(c, c->p[o + col]);
                     real sdr = cpd * (T_ref(c, P) - T0) + Phi_ref(c, P);
                     sd[col] = (cpd * (c->T[o + col] - T0) + c->Phic[k]) - sdr;
                 }
                 op_grad(c, sd, q1, q2);
+                ORA_PAR_FOR
                 for (int col = 0; col < nc; ++col) to_contra(c, col, c->sc[k], q1[col], q2[col], &v1[col], &v2[col]);
                 op_wdiv(c, c->Jc + o, v1, v2, c->Ls + o);
             }
@@ -829,6 +875,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
                 int64_t o = (int64_t)k * nc;
                 real *gd1 = c->t[8], *gd2 = c->t[8] + nc, *wc1 = c->t[9], *wc2 = c->t[9] + nc, *wc3 = c->t[10];
                 vector_laplacian_parts(c, k, c->L1 + o, c->L2 + o, c->L3 + o, gd1, gd2, wc1, wc2, wc3);
+                ORA_PAR_FOR
                 for (int col = 0; col < nc; ++col) {
                     real d1 = dd * gd1[col] - wc1[col];
                     real d2 = dd * gd2[col] - wc2[col];
@@ -839,6 +886,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
                 }
                 real *q1 = c->t[0], *q2 = c->t[0] + nc, *v1 = c->t[1], *v2 = c->t[1] + nc, *w = c->t[2];
                 op_grad(c, c->Ls + o, q1, q2);
+                ORA_PAR_FOR
                 for (int col = 0; col < nc; ++col) {
                     real a1, a2;
                     to_contra(c, col, c->sc[k], q1[col], q2[col], &a1, &a2);
@@ -846,9 +894,11 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
                     v2[col] = rho[o + col] * a2;
                 }
                 op_wdiv(c, c->Jc + o, v1, v2, w);
+                ORA_PAR_FOR
                 for (int col = 0; col < nc; ++col) RHOE(Yt)[o + col] -= ns * w[col];
             }
             /* u_3 -= nu_v * winterp(J rho, C3(nabla^4 u)) */
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col)
                 for (int f = 0; f <= nl; ++f) {
                     real v;
@@ -870,12 +920,14 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
         int64_t o = (int64_t)k * nc;
         if (sphere) op_wcurl3(c, c->Jc + o, u1 + o, u2 + o, om3 + o);
         else
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) om3[o + col] = 0;
     }
     /* omega^12 at faces: curl_v(u_h) + CT12(wcurl(u_3)) */
     real *w12a = c->t[6], *w12b = c->t[7];
     for (int f = 0; f <= nl; ++f) {
         int64_t o = (int64_t)f * nc;
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             real a = 0, b = 0;
             if (f >= 1 && f <= nl - 1) {
@@ -889,6 +941,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
         if (sphere) {
             real *x = c->t[4], *y = c->t[4] + nc;
             op_wcurl12(c, c->Jf + o, u3 + o, x, y);
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 w12a[o + col] += x[col];
                 w12b[o + col] += y[col];
@@ -900,12 +953,14 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
     real *Ga = c->t[4], *Gb = c->t[5];
     real *ct1 = c->t[2], *ct2 = c->t[3]; /* CT12(c_u) at centres */
     for (int k = 0; k < nl; ++k)
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             int64_t q = IDX(k, col);
             to_contra(c, col, c->sc[k], u1[q], u2[q], &ct1[q], &ct2[q]);
         }
     (void)G;
     for (int f = 0; f <= nl; ++f)
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             int64_t q = IDX(f, col);
             real sf = c->sf[f];
@@ -936,6 +991,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
             U3(Yt)[q] -= cr3 + gK;
         }
     for (int k = 0; k < nl; ++k)
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             int64_t q = IDX(k, col);
             real rJ = rho[q] * c->Jc[q];
@@ -952,6 +1008,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
     if (c->n.rayleigh && c->n.alpha_uh != 0)
         for (int k = 0; k < nl; ++k) {
             real beta = (real)c->n.alpha_uh * rayleigh_zeta(c, c->zc[k]);
+            ORA_PAR_FOR
             for (int col = 0; col < nc; ++col) {
                 UH1(Yt)[IDX(k, col)] += -beta * u1[IDX(k, col)];
                 UH2(Yt)[IDX(k, col)] += -beta * u2[IDX(k, col)];
@@ -963,6 +1020,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
         real cpd = (real)c->n.cp_d, T0 = (real)c->n.T_0;
         real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
         const double *B = c->g.basis;
+        ORA_PAR_FOR
         for (int col = 0; col < nc; ++col) {
             real Kh = (real)c->n.D0_col[col];
             real Km = Kh > eps ? Kh : eps;
@@ -1080,6 +1138,7 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
         remaining_tendency(c, c->FE[0], u);
         for (int i = 1; i < 4; ++i) {
             real *Ut = c->Ut, *U = c->U;
+            ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) {
                 real s = 0;
                 for (int j = 0; j < i; ++j) {
@@ -1094,21 +1153,26 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
             /* Newton iteration: cache_imp!, T_imp!, Wfact, ldiv! */
             set_precomputed(c, U);
             implicit_tendency(c, c->Rv, U);
+            ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) c->Rv[q] = Ut[q] + dtg * c->Rv[q] - U[q];
             update_jacobian(c, U, dtg);
             jac_ldiv(c, c->dY, c->Rv);
+            ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) U[q] -= c->dY[q];
             /* post-Newton correction (T_post_imp!) after a cache_imp! refresh */
             if (c->n.upwinding != 0) {
                 set_precomputed(c, U);
                 correct_implicit(c, c->dY, U);
+                ORA_PAR_FOR
                 for (int64_t q = 0; q < S; ++q) U[q] += dtg * c->dY[q];
             }
+            ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) c->FI[i][q] = (U[q] - Ut[q]) / dtg;
             /* end of stage: cache! then T_exp! */
             set_precomputed(c, U);
             remaining_tendency(c, c->FE[i], U);
         }
+        ORA_PAR_FOR
         for (int64_t q = 0; q < S; ++q) {
             real s = 0;
             for (int j = 0; j < 4; ++j)

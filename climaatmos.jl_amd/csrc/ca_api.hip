@@ -159,7 +159,7 @@ inline unsigned dss_grid(const DevGeom &g, int chunk) {
 // copy src -> dst with the impenetrability filter (columns without DSS)
 ca_status filter_columns(ca_handle *h, const float *src, float *dst, hipStream_t st) {
     const DevGeom &g = h->g;
-    k_dss_gather<<<dss_grid(g, g.nlev), 256, 0, st>>>(g, src, dst, 4, -1, 1, 1, g.nlev);
+    k_dss_gather<4, 1, true><<<dss_grid(g, g.nlev), 256, 0, st>>>(g, src, dst, 1, g.nlev);
     return check_launch(h, "filter");
 }
 
@@ -169,14 +169,14 @@ ca_status dss_gather_state(ca_handle *h, const float *src, float *dst, int filte
     const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
     {
         Launch l(h, st, kid);
-        k_dss_gather<<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, 4, 1, 1, filter, ch);
+        k_dss_gather<4, 1, true><<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
     }
     return check_launch(h, "k_dss_gather(state)");
 }
 
 // in-place state DSS (dss! hook): copy to the scratch buffer, gather back
 ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
-    if (h->g.nelem == 0 || h->g.ndss == 0) return CA_OK;
+    if (h->g.nelem == 0) return CA_OK;
     HIPCHK(h, hipMemcpyAsync(h->raw, Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
     return dss_gather_state(h, h->raw, Y, 0, st, kid);
 }
@@ -188,7 +188,7 @@ ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
 ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st, bool dss_out = false) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
-    const bool dss = n.sphere && g.ndss > 0;
+    const bool dss = n.sphere;
     if (n.sphere && n.hyper) {
         {
             Launch l(h, st, KID_LAPLACIANS);
@@ -197,7 +197,7 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
         {
             Launch l(h, st, KID_DSS_LAPLACIAN);
             const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
-            k_dss_gather<<<dss_grid(g, ch), 256, 0, st>>>(g, h->raw, h->L, 4, 0, 0, 0, ch);
+            k_dss_gather<4, 0, false><<<dss_grid(g, ch), 256, 0, st>>>(g, h->raw, h->L, 0, ch);
         }
     }
     float *out = (dss_out && dss) ? h->raw : Yt;
@@ -290,10 +290,11 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         return v;
     };
     ca_status s;
-    const float **cp[] = {&g.Jh, &g.gu11, &g.gu12, &g.gu22, &g.gd11, &g.gd12, &g.gd22, &g.f3n, &g.f12a, &g.f12b};
-    std::vector<float> colv(10 * (size_t)nc);
+    const float **cp[] = {&g.Jh, &g.gu11, &g.gu12, &g.gu22, &g.gd11, &g.gd12, &g.gd22, &g.f3n, &g.f12a, &g.f12b, &g.iJh};
+    std::vector<float> colv(11 * (size_t)nc);
     for (int i = 0; i < nc; ++i) {
         colv[0 * (size_t)nc + i] = (float)geom->J_h[i];
+        colv[10 * (size_t)nc + i] = (float)(1.0 / geom->J_h[i]);
         for (int t = 0; t < 3; ++t) {
             colv[(1 + t) * (size_t)nc + i] = (float)geom->gup[(size_t)t * nc + i];
             colv[(4 + t) * (size_t)nc + i] = (float)geom->gdn[(size_t)t * nc + i];
@@ -304,13 +305,13 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     }
     float *dcol = nullptr;
     if ((s = upload(h, &dcol, colv))) return s;
-    for (int t = 0; t < 10; ++t) *cp[t] = dcol + (size_t)t * nc;
+    for (int t = 0; t < 11; ++t) *cp[t] = dcol + (size_t)t * nc;
     float *dbasis = nullptr;
     if ((s = upload(h, &dbasis, tofloat(geom->basis, 12 * (size_t)nc)))) return s;
     g.basis = dbasis;
     // vertical tables (computed in double, stored in float)
     std::vector<double> zf(geom->z_face, geom->z_face + nl + 1);
-    std::vector<float> lc(5 * (size_t)nl), lf(5 * (size_t)(nl + 1));
+    std::vector<float> lc(8 * (size_t)nl), lf(9 * (size_t)(nl + 1));
     const double R = geom->radius;
     const double zmax = zf[nl];
     std::vector<double> zcd(nl);
@@ -326,6 +327,10 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         lc[2 * (size_t)nl + k] = (float)(R > 0 ? (R + zcd[k]) / R : 1.0);
         lc[3 * (size_t)nl + k] = (float)zeta(zcd[k]);
         lc[4 * (size_t)nl + k] = (float)(par->grav * zcd[k]);
+        const double sk = R > 0 ? (R + zcd[k]) / R : 1.0, Jv = zf[k + 1] - zf[k];
+        lc[5 * (size_t)nl + k] = (float)(1.0 / (sk * sk));
+        lc[6 * (size_t)nl + k] = (float)(1.0 / Jv);
+        lc[7 * (size_t)nl + k] = (float)(1.0 / (sk * sk * Jv));
     }
     const int nf = nl + 1;
     for (int f = 0; f <= nl; ++f) {
@@ -335,6 +340,11 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         lf[2 * (size_t)nf + f] = (float)(R > 0 ? (R + zf[f]) / R : 1.0);
         lf[3 * (size_t)nf + f] = (float)(f >= 1 && f < nl ? par->grav * (zcd[f] - zcd[f - 1]) : 0.0);
         lf[4 * (size_t)nf + f] = (float)zeta(zf[f]);
+        const double sf = R > 0 ? (R + zf[f]) / R : 1.0;
+        lf[5 * (size_t)nf + f] = (float)(1.0 / (Jvf * Jvf));
+        lf[6 * (size_t)nf + f] = (float)(1.0 / Jvf);
+        lf[7 * (size_t)nf + f] = (float)(1.0 / sf);
+        lf[8 * (size_t)nf + f] = (float)(1.0 / (sf * sf * Jvf));
     }
     std::vector<double> dphid(nl + 1, 0.0), phicd(nl);
     for (int f = 1; f < nl; ++f) dphid[f] = par->grav * (zcd[f] - zcd[f - 1]);
@@ -353,11 +363,18 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     g.sc = dlc + 2 * nl;
     g.zeta_c = dlc + 3 * nl;
     g.Phic = dlc + 4 * nl;
+    g.is2c = dlc + 5 * nl;
+    g.iJvc = dlc + 6 * nl;
+    g.isJc = dlc + 7 * nl;
     g.zf = dlf;
     g.Jvf = dlf + nf;
     g.sf = dlf + 2 * nf;
     g.dPhif = dlf + 3 * nf;
     g.zeta_f = dlf + 4 * nf;
+    g.g33f = dlf + 5 * nf;
+    g.iJvf = dlf + 6 * nf;
+    g.isf = dlf + 7 * nf;
+    g.isJf = dlf + 8 * nf;
     g.aw_col = nullptr;
     g.D0_col = nullptr;
     if (cfg->alpha_w_col) {
@@ -370,56 +387,39 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         if ((s = upload(h, &p, tofloat(cfg->D0_col, nc)))) return s;
         g.D0_col = p;
     }
-    g.ndss = geom->nelem > 0 ? geom->n_dss : 0;
-    g.col_group = nullptr;
-    if (g.ndss > 0) {
-        int m = geom->dss_ptr[g.ndss];
-        std::vector<int> ptr(geom->dss_ptr, geom->dss_ptr + g.ndss + 1), cols(geom->dss_col, geom->dss_col + m);
+    // DSS gather table: a row per column, the copies of its node in the
+    // group's canonical order, padded to four slots with (own column, 0)
+    {
+        const int ngrp = geom->nelem > 0 ? geom->n_dss : 0;
         std::vector<int> cgrp(nc, -1);
-        for (int gi = 0; gi < g.ndss; ++gi)
-            for (int t = ptr[gi]; t < ptr[gi + 1]; ++t) {
-                int c = cols[t];
-                if (c < 0 || c >= nc || cgrp[c] != -1) {
+        for (int gi = 0; gi < ngrp; ++gi)
+            for (int t = geom->dss_ptr[gi]; t < geom->dss_ptr[gi + 1]; ++t) {
+                const int c = geom->dss_col[t];
+                if (c < 0 || c >= nc || cgrp[c] != -1 || geom->dss_ptr[gi + 1] - geom->dss_ptr[gi] > 4) {
                     delete h;
                     return CA_ERR_ARG;
                 }
                 cgrp[c] = gi;
             }
-        int *dp = nullptr, *dcl = nullptr, *dg = nullptr;
-        float *dw = nullptr;
-        if ((s = upload(h, &dp, ptr))) return s;
-        if ((s = upload(h, &dcl, cols))) return s;
-        if ((s = upload(h, &dw, tofloat(geom->dss_w, m)))) return s;
-        if ((s = upload(h, &dg, cgrp))) return s;
-        g.dptr = dp;
-        g.dcol = dcl;
-        g.dw = dw;
-        g.col_group = dg;
-        // per-column gather table (slot 0 = own column), computed in double
-        std::vector<int> gidx(4 * (size_t)nc), grank(nc, 0), gmv(nc, 1);
-        std::vector<float> gw(4 * (size_t)nc, 0.0f), gM(16 * (size_t)nc, 0.0f);
+        std::vector<int> tc(4 * (size_t)nc);
+        std::vector<float> tw(4 * (size_t)nc, 0.0f), tM(16 * (size_t)nc, 0.0f);
         const double *Bs = geom->basis;
         auto bas = [&](int plane, int c) { return Bs[(size_t)plane * nc + c]; };
         for (int c = 0; c < nc; ++c) {
-            for (int t = 0; t < 4; ++t) gidx[(size_t)t * nc + c] = c;
-            gw[c] = 1.0f;
-            gM[0 * (size_t)nc + c] = 1.0f;
-            gM[3 * (size_t)nc + c] = 1.0f;
+            for (int t = 0; t < 4; ++t) tc[4 * (size_t)c + t] = c;
             const int gi = cgrp[c];
-            if (gi < 0) continue;
-            const int b0 = ptr[gi], m = ptr[gi + 1] - b0;
-            if (m > 4) {
-                delete h;
-                return CA_ERR_ARG;
+            if (gi < 0) {
+                tw[4 * (size_t)c] = 1.0f;
+                tM[16 * (size_t)c + 0] = 1.0f;
+                tM[16 * (size_t)c + 3] = 1.0f;
+                continue;
             }
-            int slot = 1, rk = 0;
-            for (int r = 0; r < m; ++r) {
-                const int mcol = cols[b0 + r];
-                const int t = mcol == c ? 0 : slot++;
-                gidx[(size_t)t * nc + c] = mcol;
-                gw[(size_t)t * nc + c] = (float)geom->dss_w[b0 + r];
-                rk |= r << (2 * t);
-                const double w = geom->dss_w[b0 + r];
+            const int b0 = geom->dss_ptr[gi], m = geom->dss_ptr[gi + 1] - b0;
+            for (int t = 0; t < m; ++t) {
+                const int mcol = geom->dss_col[b0 + t];
+                const double w = geom->dss_w[b0 + t];
+                tc[4 * (size_t)c + t] = mcol;
+                tw[4 * (size_t)c + t] = (float)w;
                 double m00 = 0, m01 = 0, m10 = 0, m11 = 0;
                 for (int d = 0; d < 3; ++d) {
                     m00 += bas(6 + d, c) * bas(d, mcol);
@@ -427,36 +427,32 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
                     m10 += bas(9 + d, c) * bas(d, mcol);
                     m11 += bas(9 + d, c) * bas(3 + d, mcol);
                 }
-                gM[(size_t)(4 * t + 0) * nc + c] = (float)(w * m00);
-                gM[(size_t)(4 * t + 1) * nc + c] = (float)(w * m01);
-                gM[(size_t)(4 * t + 2) * nc + c] = (float)(w * m10);
-                gM[(size_t)(4 * t + 3) * nc + c] = (float)(w * m11);
+                float *Mt = &tM[16 * (size_t)c + 4 * t];
+                Mt[0] = (float)(w * m00);
+                Mt[1] = (float)(w * m01);
+                Mt[2] = (float)(w * m10);
+                Mt[3] = (float)(w * m11);
             }
-            grank[c] = rk;
-            gmv[c] = m;
         }
-        int *d_gidx = nullptr, *d_grank = nullptr, *d_gm = nullptr;
-        float *d_gw = nullptr, *d_gM = nullptr;
-        if ((s = upload(h, &d_gidx, gidx))) return s;
-        if ((s = upload(h, &d_grank, grank))) return s;
-        if ((s = upload(h, &d_gm, gmv))) return s;
-        if ((s = upload(h, &d_gw, gw))) return s;
-        if ((s = upload(h, &d_gM, gM))) return s;
-        g.gidx = d_gidx;
-        g.grank = d_grank;
-        g.gm = d_gm;
-        g.gw = d_gw;
-        g.gM = d_gM;
-    } else {
-        g.gidx = g.grank = g.gm = nullptr;
-        g.gw = g.gM = nullptr;
+        int *d_tc = nullptr;
+        float *d_tw = nullptr, *d_tM = nullptr;
+        if ((s = upload(h, &d_tc, tc))) return s;
+        if ((s = upload(h, &d_tw, tw))) return s;
+        if ((s = upload(h, &d_tM, tM))) return s;
+        g.gtc = reinterpret_cast<const int4 *>(d_tc);
+        g.gtw = reinterpret_cast<const float4 *>(d_tw);
+        g.gtM = reinterpret_cast<const float4 *>(d_tM);
     }
-    float W[16], Dm[4][4];
+    float W[16], iW[16], Dm[4][4];
     for (int j = 0; j < 4; ++j)
-        for (int i = 0; i < 4; ++i) W[4 * j + i] = (float)(geom->gll_w[i] * geom->gll_w[j]);
+        for (int i = 0; i < 4; ++i) {
+            W[4 * j + i] = (float)(geom->gll_w[i] * geom->gll_w[j]);
+            iW[4 * j + i] = (float)(1.0 / (geom->gll_w[i] * geom->gll_w[j]));
+        }
     for (int i = 0; i < 4; ++i)
         for (int k = 0; k < 4; ++k) Dm[i][k] = (float)geom->gll_D[4 * i + k];
     HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_W), W, sizeof(W)));
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_iW), iW, sizeof(iW)));
     HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_D), Dm, sizeof(Dm)));
     DevNum &n = h->n;
     n.R_d = (float)par->R_d;
@@ -473,6 +469,7 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     n.dR_d = par->R_d;
     n.dcp_d = par->cp_d;
     n.dcv_d = par->cv_d;
+    n.dicv_d = 1.0 / par->cv_d;
     n.dT_0 = par->T_0;
     n.dp0inv = 1.0 / par->p_ref_theta;
     n.dkex = par->R_d / par->cp_d;
@@ -599,7 +596,7 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
     ca_status s;
     // the step starts from a continuous, filtered state (the reference's
     // state after dss! and cache! at the end of every step)
-    if (g.nelem > 0 && g.ndss > 0) {
+    if (g.nelem > 0) {
         HIPCHK(h, hipMemcpyAsync(h->U, Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
         if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
     } else {
@@ -661,7 +658,7 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
             k_combine<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(g, Y, h->U, fs);
         }
         if ((s = check_launch(h, "k_combine"))) return s;
-        if (g.nelem > 0 && g.ndss > 0) {
+        if (g.nelem > 0) {
             if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
         } else {
             if ((s = filter_columns(h, h->U, Y, st))) return s;

@@ -27,21 +27,21 @@ __device__ __forceinline__ CV cv_make(const DevGeom &g, const DevNum &n, int k, 
     c.u2 = u2;
     c.rhoe = rhoe;
     float s = g.sc[k];
-    float is2 = 1.0f / (s * s);
+    float is2 = g.is2c[k];
     float ct1, ct2;
     ca_contra(g, col, is2, u1, u2, ct1, ct2);
     c.ct1 = ct1;
     c.ct2 = ct2;
-    float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]);
-    float gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
+    float ga = g.g33f[k];
+    float gb = g.g33f[k + 1];
     float vv = 0.5f * (u3a * ga * u3a + u3b * gb * u3b);
     c.K = 0.5f * ((u1 * ct1 + u2 * ct2) + vv);
     // thermodynamics in float64: the implicit pressure gradient is a small
     // difference of large hydrostatic terms, so T and p are kept to double
     // precision for it (the state itself stays float32)
-    double e_tot = (double)rhoe / (double)rho;
+    double e_tot = (double)rhoe * ca_drcp((double)rho);
     double e_int = e_tot - (double)c.K - g.Phic_d[k];
-    double Td = n.dT_0 + (e_int + n.dR_d * n.dT_0) / n.dcv_d;
+    double Td = n.dT_0 + (e_int + n.dR_d * n.dT_0) * n.dicv_d;
     Td = Td > n.dTmin ? Td : n.dTmin;
     double pd = (double)rho * n.dR_d * Td;
     c.Td = Td;
@@ -52,6 +52,7 @@ __device__ __forceinline__ CV cv_make(const DevGeom &g, const DevNum &n, int k, 
     c.p = (float)pd;
     c.Pi = (float)c.Pid;
     c.J = g.Jh[col] * s * s * g.Jvc[k];
+    c.iJ = g.iJh[col] * g.isJc[k];
     return c;
 }
 
@@ -79,15 +80,15 @@ __device__ __forceinline__ float upwind_product(int scheme, int f, int nlev, flo
 // above (b): -(grad Phi - grad Phi_r + cp_d I^f(theta_v - theta_vr) grad Pi)
 // (implicit_tendency.jl:287-293), written with cancellation-free differences.
 __device__ __forceinline__ float pgf_u3(const DevGeom &g, const DevNum &n, int f, const CV &a, const CV &b) {
-    double lr = n.dkex * log1p((b.pd - a.pd) / a.pd);
+    double lr = n.dkex * log1p((b.pd - a.pd) * ca_drcp(a.pd));
     double Pa = a.Pid;
     double Pa7 = Pa * Pa * Pa * Pa * Pa * Pa * Pa;
     double dPi = Pa * expm1(lr);
     double dPhr = -n.dcp_d * (n.dTmr * lr + (n.dTsr - n.dTmr) * (1.0 / 7.0) * Pa7 * expm1(7.0 * lr));
     double Pb = Pa + dPi;
     double Pb7 = Pb * Pb * Pb * Pb * Pb * Pb * Pb;
-    double tha = (a.Td - (n.dTmr + (n.dTsr - n.dTmr) * Pa7)) / Pa;
-    double thb = (b.Td - (n.dTmr + (n.dTsr - n.dTmr) * Pb7)) / Pb;
+    double tha = (a.Td - (n.dTmr + (n.dTsr - n.dTmr) * Pa7)) * ca_drcp(Pa);
+    double thb = (b.Td - (n.dTmr + (n.dTsr - n.dTmr) * Pb7)) * ca_drcp(Pb);
     return (float)(-((g.dPhif_d[f] - dPhr) + n.dcp_d * (0.5 * (tha + thb)) * dPi));
 }
 
@@ -166,15 +167,15 @@ __device__ __forceinline__ void jac_center_row(const DevGeom &g, int k, const CV
     a0 = a1 = e0 = e1 = 0.0f;
     if (cm) {
         float rJ = 0.5f * (cm->rho * cm->J + c.rho * c.J);
-        float g33 = 1.0f / (g.Jvf[k] * g.Jvf[k]);
-        float adv = rJ / c.J;
+        float g33 = g.g33f[k];
+        float adv = rJ * c.iJ;
         a0 = dtg * adv * g33;
         e0 = a0 * (0.5f * (cm->h + c.h));
     }
     if (cp) {
         float rJ = 0.5f * (c.rho * c.J + cp->rho * cp->J);
-        float g33 = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
-        float adv = -rJ / c.J;
+        float g33 = g.g33f[k + 1];
+        float adv = -rJ * c.iJ;
         a1 = dtg * adv * g33;
         e1 = a1 * (0.5f * (c.h + cp->h));
     }
@@ -188,15 +189,15 @@ __device__ __forceinline__ FaceRow jac_face_row(const DevGeom &g, const DevNum &
                                                 const CV &b, float u3m, float u3f, float u3p, float dtg) {
     FaceRow r;
     float rf = 0.5f * (a.rho + b.rho);
-    float pg_m = 1.0f / rf, pg_p = -1.0f / rf;
+    float pg_m = ca_rcp(rf), pg_p = -pg_m;
     float kap = n.kap;
     // d p / d rho at fixed rho e_tot (manual_sparse_jacobian.jl:816-818); the
     // (R_d - kappa cv_d) T term vanishes for dry air.
     float dpa = kap * (n.T_0 * n.cp_d - a.K - g.Phic[f - 1]);
     float dpb = kap * (n.T_0 * n.cp_d - b.K - g.Phic[f]);
-    float thva = a.T / a.Pi, thvb = b.T / b.Pi;
-    float dPi = ca_dExner(n, a.p, a.Pi, b.p);
-    float buoy = n.cp_d * (0.5f * (thva + thvb)) * dPi / rf;
+    float thva = a.T * ca_rcp(a.Pi), thvb = b.T * ca_rcp(b.Pi);
+    float dPi = ca_dExner(n, a.p, ca_rcp(a.p), a.Pi, b.p);
+    float buoy = n.cp_d * (0.5f * (thva + thvb)) * dPi * pg_m;
     r.r0 = dtg * (pg_m * dpa + 0.5f * buoy);
     r.r1 = dtg * (pg_p * dpb + 0.5f * buoy);
     r.e0 = dtg * (pg_m * kap);
@@ -206,9 +207,9 @@ __device__ __forceinline__ FaceRow jac_face_row(const DevGeom &g, const DevNum &
     r.h0b = dtg * pg_m * ka * a.ct2;
     r.h1a = dtg * pg_p * kb * b.ct1;
     r.h1b = dtg * pg_p * kb * b.ct2;
-    float gm = 1.0f / (g.Jvf[f - 1] * g.Jvf[f - 1]);
-    float g0 = 1.0f / (g.Jvf[f] * g.Jvf[f]);
-    float gp = 1.0f / (g.Jvf[f + 1] * g.Jvf[f + 1]);
+    float gm = g.g33f[f - 1];
+    float g0 = g.g33f[f];
+    float gp = g.g33f[f + 1];
     float beta = beta_w(g, n, f, col);
     r.lo = dtg * (pg_m * ka * (0.5f * u3m * gm));
     r.d = dtg * (pg_m * ka * (0.5f * u3f * g0) + pg_p * kb * (0.5f * u3f * g0) - beta) - 1.0f;

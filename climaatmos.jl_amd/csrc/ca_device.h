@@ -14,6 +14,7 @@
 // GLL Nq = 4: quadrature weight products W[4 j + i] = w_i w_j and the
 // derivative matrix D[i][k] = l_k'(xi_i).  Filled once by ca_create.
 __constant__ float c_W[CA_NN];
+__constant__ float c_iW[CA_NN];  // 1 / W
 __constant__ float c_D[CA_NQ][CA_NQ];
 
 struct DevGeom {
@@ -21,31 +22,36 @@ struct DevGeom {
     int64_t nC, nF;  // centre / face plane sizes
     // per-column planes (ncol each)
     const float *Jh, *gu11, *gu12, *gu22, *gd11, *gd12, *gd22, *f3n, *f12a, *f12b;
+    const float *iJh;                    // 1 / J_h
     const float *basis;  // 12 planes: a^1 (x,y,z), a^2, a_1, a_2
     // per-level tables
     const float *zc, *Jvc, *sc, *dzc;    // centres [nlev]; dzc = Jvc
     const float *zf, *Jvf, *sf, *dPhif;  // faces [nlev + 1]; dPhif[f] = g (zc[f] - zc[f-1])
     const float *zeta_c, *zeta_f;        // Rayleigh profiles sin^2(...) (0 below zd)
     const float *Phic;                   // g zc
+    // reciprocals (per level, computed in double): 1 / s_c^2, 1 / Jv_c,
+    // 1 / (s_c^2 Jv_c); g^33 = 1 / Jv_f^2, 1 / Jv_f, 1 / s_f, 1 / (s_f^2 Jv_f)
+    const float *is2c, *iJvc, *isJc;
+    const float *g33f, *iJvf, *isf, *isJf;
     const double *dPhif_d;               // dPhif in float64
     const double *Phic_d;                // g zc in float64
-    const int *col_group;                // shared-node group of a column (-1: interior node)
-    // per-column gather table of the DSS, slot-major planes of ncol: slot 0 is
-    // the column itself; gidx[4] member columns, gw[4] weights, grank packs the
-    // canonical (group-order) rank of each slot in 2 bits, gM[16] the 2x2
-    // covariant transform w_s (a_i(own) . a^j(slot)) of each slot, gm = count
-    const int *gidx, *grank, *gm;
-    const float *gw, *gM;
+    // per-column gather table of the weighted DSS, one row per column, the
+    // copies of the column's node in the group's canonical order, padded to
+    // four slots with (own column, weight 0):
+    //   gtc[col]  int4    column of each slot
+    //   gtw[col]  float4  weight WJ / sum WJ of each slot
+    //   gtM[col]  float4 x 4, slot t: w_t (a_1 . a^1, a_1 . a^2, a_2 . a^1, a_2 . a^2)
+    //             with a_i of the own column and a^j of the slot's column
+    // An interior node is its own single slot of weight 1.
+    const int4 *gtc;
+    const float4 *gtw, *gtM;
     const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
-    // DSS groups
-    int ndss;
-    const int *dptr, *dcol;
-    const float *dw;
 };
 
 struct DevNum {
     float R_d, cp_d, cv_d, T_0, p0inv, kex, kap, grav, Tsr, Tmr, Tmin;
     double dR_d, dcp_d, dcv_d, dT_0, dp0inv, dkex, dTsr, dTmr, dTmin;
+    double dicv_d;  // 1 / cv_d
     float nu_v, nu_s, dd;
     float alpha_uh, alpha_w, dt;
     int hyper, upw, rayleigh, vdiff, sphere;
@@ -59,7 +65,19 @@ __device__ __forceinline__ float ca_temperature(const DevNum &n, float e_int) {
     float T = n.T_0 + (e_int + n.R_d * n.T_0) / n.cv_d;
     return fmaxf(T, n.Tmin);
 }
-__device__ __forceinline__ float ca_exner(const DevNum &n, float p) { return powf(p * n.p0inv, n.kex); }
+// Exner function (p / p0)^kex through the hardware log2 / exp2 (p > 0)
+__device__ __forceinline__ float ca_exner(const DevNum &n, float p) {
+    return __builtin_amdgcn_exp2f(n.kex * __builtin_amdgcn_logf(p * n.p0inv));
+}
+// 1 / x to about 1 ulp (v_rcp_f32)
+__device__ __forceinline__ float ca_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1 / x in float64 (finite, normal x): v_rcp_f64 and two Newton-Raphson steps
+__device__ __forceinline__ double ca_drcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
 // reference temperature T_r(Pi) = T_min + (T_sfc - T_min) Pi^7 (refstate_thermodynamics.jl:80-88)
 __device__ __forceinline__ float ca_pow7(float x) {
     float x2 = x * x;
@@ -70,13 +88,14 @@ __device__ __forceinline__ float ca_Tref(const DevNum &n, float Pi) { return n.T
 // Phi_r(p_b) - Phi_r(p_a) in a cancellation-free form:
 //   -cp_d [T_min ln(Pi_b / Pi_a) + (T_sfc - T_min) / 7 (Pi_b^7 - Pi_a^7)]
 // with ln(Pi_b / Pi_a) = kex ln(p_b / p_a) and Pi_b^7 - Pi_a^7 = Pi_a^7 expm1(7 ln(Pi_b / Pi_a)).
-__device__ __forceinline__ float ca_dPhiref(const DevNum &n, float pa, float Pia, float pb) {
-    float lr = n.kex * log1pf((pb - pa) / pa);
+// ipa = 1 / pa
+__device__ __forceinline__ float ca_dPhiref(const DevNum &n, float pa, float ipa, float Pia, float pb) {
+    float lr = n.kex * log1pf((pb - pa) * ipa);
     return -n.cp_d * (n.Tmr * lr + (n.Tsr - n.Tmr) * (1.0f / 7.0f) * ca_pow7(Pia) * expm1f(7.0f * lr));
 }
 // Pi_b - Pi_a = Pi_a expm1(kex ln(p_b / p_a))
-__device__ __forceinline__ float ca_dExner(const DevNum &n, float pa, float Pia, float pb) {
-    return Pia * expm1f(n.kex * log1pf((pb - pa) / pa));
+__device__ __forceinline__ float ca_dExner(const DevNum &n, float pa, float ipa, float Pia, float pb) {
+    return Pia * expm1f(n.kex * log1pf((pb - pa) * ipa));
 }
 
 // horizontal metric at scale s (no topography): contravariant u^a = g^{ab} u_b / s^2
@@ -159,47 +178,60 @@ __device__ __forceinline__ float ca_d2t(const Nod &q, float f) {
     return fmaf(q.c2t[3], ca_dpp<CA_ROR(12)>(f), r);
 }
 
+// Metric factors of one node at one level: W = quadrature weight product,
+// J = 3-D Jacobian, and the reciprocals the operators multiply by.
+struct Met {
+    float W, iW, J, iJ, WJ, iWJ;
+};
+__device__ __forceinline__ Met ca_met(float W, float iW, float J, float iJ) {
+    Met m;
+    m.W = W;
+    m.iW = iW;
+    m.J = J;
+    m.iJ = iJ;
+    m.WJ = W * J;
+    m.iWJ = iW * iJ;
+    return m;
+}
+
 // Spectral-element operators at one level (docs/src/discretization.md:72-135).
-// J = 3-D Jacobian at the node, W = quadrature weight product.
 // strong gradient -> C12
 __device__ __forceinline__ void op_grad(const Nod &q, float f, float &o1, float &o2) {
     o1 = ca_d1(q, f);
     o2 = ca_d2(q, f);
 }
-// strong divergence of contravariant (v1, v2)
-__device__ __forceinline__ float op_div(const Nod &q, float J, float v1, float v2) {
-    return (ca_d1(q, J * v1) + ca_d2(q, J * v2)) / J;
+// strong divergence of contravariant (v1, v2): (D (J v1) + D (J v2)) / J
+__device__ __forceinline__ float op_div(const Nod &q, const Met &m, float v1, float v2) {
+    return (ca_d1(q, m.J * v1) + ca_d2(q, m.J * v2)) * m.iJ;
 }
 // weak divergence: -(D^T (W J v1) + D^T (W J v2)) / (W J)
-__device__ __forceinline__ float op_wdiv(const Nod &q, float W, float J, float v1, float v2) {
-    float WJ = W * J;
-    return -(ca_d1t(q, WJ * v1) + ca_d2t(q, WJ * v2)) / WJ;
+__device__ __forceinline__ float op_wdiv(const Nod &q, const Met &m, float v1, float v2) {
+    return -(ca_d1t(q, m.WJ * v1) + ca_d2t(q, m.WJ * v2)) * m.iWJ;
 }
 // weak gradient -> C12: -(D^T (W f)) / W
-__device__ __forceinline__ void op_wgrad(const Nod &q, float W, float f, float &o1, float &o2) {
-    float Wf = W * f;
-    o1 = -ca_d1t(q, Wf) / W;
-    o2 = -ca_d2t(q, Wf) / W;
+__device__ __forceinline__ void op_wgrad(const Nod &q, const Met &m, float f, float &o1, float &o2) {
+    const float Wf = m.W * f;
+    o1 = -ca_d1t(q, Wf) * m.iW;
+    o2 = -ca_d2t(q, Wf) * m.iW;
 }
 // strong curl of C12 (u1, u2) -> CT3
-__device__ __forceinline__ float op_curl3(const Nod &q, float J, float u1, float u2) {
-    return (ca_d1(q, u2) - ca_d2(q, u1)) / J;
+__device__ __forceinline__ float op_curl3(const Nod &q, const Met &m, float u1, float u2) {
+    return (ca_d1(q, u2) - ca_d2(q, u1)) * m.iJ;
 }
 // weak curl of C12 -> CT3: (-D^T (W u2) + D^T (W u1)) / (W J)
-__device__ __forceinline__ float op_wcurl3(const Nod &q, float W, float J, float u1, float u2) {
-    return (-ca_d1t(q, W * u2) + ca_d2t(q, W * u1)) / (W * J);
+__device__ __forceinline__ float op_wcurl3(const Nod &q, const Met &m, float u1, float u2) {
+    return (-ca_d1t(q, m.W * u2) + ca_d2t(q, m.W * u1)) * m.iWJ;
 }
 // strong curl of C3 u3 -> CT12
-__device__ __forceinline__ void op_curl12(const Nod &q, float J, float u3, float &o1, float &o2) {
-    o1 = ca_d2(q, u3) / J;
-    o2 = -ca_d1(q, u3) / J;
+__device__ __forceinline__ void op_curl12(const Nod &q, const Met &m, float u3, float &o1, float &o2) {
+    o1 = ca_d2(q, u3) * m.iJ;
+    o2 = -ca_d1(q, u3) * m.iJ;
 }
 // weak curl of C3 -> CT12
-__device__ __forceinline__ void op_wcurl12(const Nod &q, float W, float J, float u3, float &o1, float &o2) {
-    float Wu = W * u3;
-    float WJ = W * J;
-    o1 = -ca_d2t(q, Wu) / WJ;
-    o2 = ca_d1t(q, Wu) / WJ;
+__device__ __forceinline__ void op_wcurl12(const Nod &q, const Met &m, float u3, float &o1, float &o2) {
+    const float Wu = m.W * u3;
+    o1 = -ca_d2t(q, Wu) * m.iWJ;
+    o2 = ca_d1t(q, Wu) * m.iWJ;
 }
 
 // van Leer (Lin et al. 1994) slope with the monotone local-extrema

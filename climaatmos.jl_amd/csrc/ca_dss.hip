@@ -3,65 +3,27 @@
 // and remaining_tendency.jl:18-21).  Each shared node's value becomes the
 // J-weighted average over its copies; covariant horizontal vectors are
 // averaged as Cartesian vectors and projected back onto each element's basis.
-// Thread per (shared-node group, level); interior nodes are untouched.
+//
+// Gather form, out of place: thread per (column, chunk of levels).  A column
+// reads every copy of its node (at most four on the cubed sphere) and writes
+// only itself.  The four slots of a column's table row list the copies in
+// the group's canonical order, padded with zero-weight slots, so
+//   * every copy of a node sums the same values in the same order with the
+//     same weights: duplicated columns receive bitwise-identical scalars;
+//   * an interior node (one slot of weight 1) passes through exactly;
+//   * control flow is uniform: each plane and level is one store of all
+//     lanes (whole cache lines); padded slots load the column's own value
+//     (always in bounds, an L1 hit) and contribute 0 * v = 0.
+// The table row is read once per thread (int4 + float4 [+ 4 float4]) and
+// reused over the thread's levels.
 #include "ca_kernels.h"
 
-__global__ void __launch_bounds__(256) k_dss_scalar(DevGeom g, float *f, int nplanes, int nlev_plane,
-                                                    int64_t plane_stride) {
-    int grp = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = blockIdx.y;
-    if (grp >= g.ndss || k >= nlev_plane) return;
-    const int b = g.dptr[grp], e = g.dptr[grp + 1];
-    const int64_t off = (int64_t)k * g.ncol;
-    for (int pl = 0; pl < nplanes; ++pl) {
-        float *fk = f + pl * plane_stride + off;
-        float s = 0.0f;
-        for (int t = b; t < e; ++t) s = fmaf(g.dw[t], fk[g.dcol[t]], s);
-        for (int t = b; t < e; ++t) fk[g.dcol[t]] = s;
-    }
-}
-
-__global__ void __launch_bounds__(256) k_dss_vector(DevGeom g, float *u1, float *u2, int nlev_plane) {
-    int grp = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = blockIdx.y;
-    if (grp >= g.ndss || k >= nlev_plane) return;
-    const int b = g.dptr[grp], e = g.dptr[grp + 1];
-    const int64_t off = (int64_t)k * g.ncol;
-    const int64_t nc = g.ncol;
-    const float *B = g.basis;
-    float X0 = 0.0f, X1 = 0.0f, X2 = 0.0f;
-    for (int t = b; t < e; ++t) {
-        int c = g.dcol[t];
-        float w = g.dw[t];
-        float a = w * u1[off + c], bb = w * u2[off + c];
-        X0 += a * B[0 * nc + c] + bb * B[3 * nc + c];
-        X1 += a * B[1 * nc + c] + bb * B[4 * nc + c];
-        X2 += a * B[2 * nc + c] + bb * B[5 * nc + c];
-    }
-    for (int t = b; t < e; ++t) {
-        int c = g.dcol[t];
-        u1[off + c] = X0 * B[6 * nc + c] + X1 * B[7 * nc + c] + X2 * B[8 * nc + c];
-        u2[off + c] = X0 * B[9 * nc + c] + X1 * B[10 * nc + c] + X2 * B[11 * nc + c];
-    }
-}
-
-// Gather form of the weighted DSS, out of place: thread per (column, chunk
-// of levels).  A column of a shared node reads every copy of its node and
-// writes only itself; a column of an interior node copies.  Loads are
-// issued own-column first (coalesced across the wavefront); the scalar sum
-// runs over the copies in the group's canonical order (the rank of each
-// slot), so every copy of a node computes bitwise the same value.
-// Covariant horizontal vector planes (vec, vec + 1) use the per-slot 2x2
-// transforms w_s (a_i(own) . a^j(slot)), i.e. the Cartesian average
-// projected back on the column's own basis.  The per-column table is read
-// once per thread (coalesced slot-major planes) and reused over its levels.
-//   ncp: centre planes; vec: index of the first of the two vector planes
-//   (-1: none); face: 1 if a face plane follows the centre planes;
-//   filter: zero the bottom / top face values (impenetrability).
-//   The 1-D grid covers (column block, level chunk) pairs; blocks are
-//   dispatched round-robin over the 8 XCDs, so the logical order is remapped
-//   to give every XCD a contiguous span of column blocks (neighbouring
-//   elements, whose copies a gather reads, then share that XCD's L2).
+// The 1-D grid covers (column block, level chunk) pairs.  Blocks are
+// dispatched round-robin over the 8 XCDs; the logical order is remapped to
+// give every XCD a contiguous span of column blocks, so neighbouring
+// elements, whose copies a gather reads, share that XCD's L2.  The host pads
+// the grid to a multiple of 8 blocks: b -> (b % 8) * q + b / 8 is then a
+// bijection onto [0, 8 q), and logical blocks past the end return.
 __device__ __forceinline__ bool xcd_block(int nb_col, int nb_lev, int &cb, int &lb) {
     const int nb = nb_col * nb_lev;
     const int q = (nb + 7) / 8;
@@ -73,28 +35,20 @@ __device__ __forceinline__ bool xcd_block(int nb_col, int nb_lev, int &cb, int &
     return true;
 }
 
-// canonical-order weighted sum of up to four slot values
-__device__ __forceinline__ float ranked_sum(const float v[4], const float w[4], int rank, int m) {
+// weighted sum over the four slots of one plane row, in slot order
+__device__ __forceinline__ float dss_scalar_row(const float *__restrict__ row, const int mc[4], const float mw[4]) {
+    float v[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) v[t] = row[mc[t]];
     float s = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if (r >= m) break;
-        // slot holding canonical rank r
-        float vv = 0.0f, ww = 0.0f;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const bool hit = ((rank >> (2 * t)) & 3) == r && t < m;
-            vv = hit ? v[t] : vv;
-            ww = hit ? w[t] : ww;
-        }
-        s = fmaf(ww, vv, s);
-    }
+    for (int t = 0; t < 4; ++t) s = fmaf(mw[t], v[t], s);
     return s;
 }
 
+template <int NC, int VEC, bool FACE>
 __global__ void __launch_bounds__(256) k_dss_gather(DevGeom g, const float *__restrict__ src,
-                                                    float *__restrict__ dst, int ncp, int vec, int face,
-                                                    int filter, int chunk) {
+                                                    float *__restrict__ dst, int filter, int chunk) {
     const int nl = g.nlev;
     int cb, lb;
     if (!xcd_block((g.ncol + 255) / 256, (nl + chunk - 1) / chunk, cb, lb)) return;
@@ -103,67 +57,50 @@ __global__ void __launch_bounds__(256) k_dss_gather(DevGeom g, const float *__re
     const int64_t nC = g.nC, nc = g.ncol;
     const int k0 = lb * chunk;
     const int k1 = min(k0 + chunk, nl);
-    const int kf1 = (k1 == nl) ? nl + 1 : k1;  // the last chunk also owns face nlev
-    const int m = g.gm ? g.gm[col] : 1;
-    if (m <= 1) {
-#pragma unroll 4
-        for (int k = k0; k < kf1; ++k) {
-            const int64_t q = (int64_t)k * nc + col;
-            if (k < nl)
-                for (int p = 0; p < ncp; ++p) dst[p * nC + q] = src[p * nC + q];
-            if (face) {
-                float v = src[ncp * nC + q];
-                if (filter && (k == 0 || k == nl)) v = 0.0f;
-                dst[ncp * nC + q] = v;
-            }
-        }
-        return;
-    }
-    int mc[4];
-    float mw[4], M[4][4];
+    const int kf1 = (FACE && k1 == nl) ? nl + 1 : k1;  // the last chunk also owns face nlev
+    const int4 ci = g.gtc[col];
+    const float4 wv = g.gtw[col];
+    const int mc[4] = {ci.x, ci.y, ci.z, ci.w};
+    const float mw[4] = {wv.x, wv.y, wv.z, wv.w};
+    float4 M[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        mc[t] = g.gidx[t * nc + col];
-        mw[t] = g.gw[t * nc + col];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) M[t][e] = vec >= 0 ? g.gM[(4 * t + e) * nc + col] : 0.0f;
-    }
-    const int rank = g.grank[col];
+    for (int t = 0; t < 4; ++t) M[t] = VEC >= 0 ? g.gtM[4 * (int64_t)col + t] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll 2
     for (int k = k0; k < kf1; ++k) {
         const int64_t row = (int64_t)k * nc;
-        if (k < nl) {
-            for (int p = 0; p < ncp; ++p) {
-                if (p == vec) {
+        if (k < k1) {
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                if (p == VEC + 1) continue;
+                const float *rp = src + p * nC + row;
+                if (p == VEC) {
+                    const float *rq = rp + nC;
+                    float a[4], b[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        a[t] = rp[mc[t]];
+                        b[t] = rq[mc[t]];
+                    }
                     float o1 = 0.0f, o2 = 0.0f;
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
-                        if (t >= m) break;
-                        const float a = src[p * nC + row + mc[t]];
-                        const float b = src[(p + 1) * nC + row + mc[t]];
-                        o1 = fmaf(M[t][0], a, fmaf(M[t][1], b, o1));
-                        o2 = fmaf(M[t][2], a, fmaf(M[t][3], b, o2));
+                        o1 = fmaf(M[t].x, a[t], fmaf(M[t].y, b[t], o1));
+                        o2 = fmaf(M[t].z, a[t], fmaf(M[t].w, b[t], o2));
                     }
                     dst[p * nC + row + col] = o1;
                     dst[(p + 1) * nC + row + col] = o2;
-                    ++p;
-                    continue;
+                } else {
+                    dst[p * nC + row + col] = dss_scalar_row(rp, mc, mw);
                 }
-                float v[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) v[t] = t < m ? src[p * nC + row + mc[t]] : 0.0f;
-                dst[p * nC + row + col] = ranked_sum(v, mw, rank, m);
             }
         }
-        if (face) {
-            float s = 0.0f;
-            if (!(filter && (k == 0 || k == nl))) {
-                float v[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) v[t] = t < m ? src[ncp * nC + row + mc[t]] : 0.0f;
-                s = ranked_sum(v, mw, rank, m);
-            }
-            dst[ncp * nC + row + col] = s;
+        if (FACE) {
+            float s = dss_scalar_row(src + NC * nC + row, mc, mw);
+            if (filter && (k == 0 || k == nl)) s = 0.0f;
+            dst[NC * nC + row + col] = s;
         }
     }
 }
+
+template __global__ void k_dss_gather<4, 1, true>(DevGeom, const float *, float *, int, int);
+template __global__ void k_dss_gather<4, 0, false>(DevGeom, const float *, float *, int, int);

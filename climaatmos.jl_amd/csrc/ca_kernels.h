@@ -6,7 +6,7 @@
 struct CV {
     float rho, u1, u2, rhoe;
     float ct1, ct2;  // contravariant horizontal velocity
-    float K, T, h, p, Pi, J;
+    float K, T, h, p, Pi, J, iJ;  // iJ = 1 / J
     double Td, pd, Pid;  // temperature, pressure and Exner function in float64
 };
 
@@ -53,9 +53,9 @@ __global__ void k_laplacians(DevGeom g, DevNum n, const float *Y, float *L);
 template <bool SPH, bool HYPER, bool VDIFF>
 __global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk);
 
-// DSS (ca_dss.hip): weighted average over shared nodes, level range of planes
-__global__ void k_dss_scalar(DevGeom g, float *f, int nplanes, int nlev_plane, int64_t plane_stride);
-__global__ void k_dss_vector(DevGeom g, float *u1, float *u2, int nlev_plane);
-__global__ void k_dss_gather(DevGeom g, const float *src, float *dst, int ncp, int vec, int face, int filter,
-                             int chunk);
+// DSS (ca_dss.hip): out-of-place weighted average over shared nodes.
+// NC centre planes, the vector pair at planes (VEC, VEC + 1) (-1: none), a
+// face plane after them if FACE.
+template <int NC, int VEC, bool FACE>
+__global__ void k_dss_gather(DevGeom g, const float *src, float *dst, int filter, int chunk);
 __global__ void k_combine(DevGeom g, const float *u, float *out, FinalSrc s);

@@ -90,10 +90,9 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
     const bool isF = k <= nl;
     const int kc = hasC ? k : nl - 1;   // clamped centre level for padding lanes
     const int kf = isF ? k : nl;        // clamped face level
-    const float Jvf = g.Jvf[kf];
-    const float g33 = 1.0f / (Jvf * Jvf);
-    const float Jvf1 = g.Jvf[kc + 1];
-    const float g33p = 1.0f / (Jvf1 * Jvf1);
+    const float g33 = g.g33f[kf];
+    const float g33p = g.g33f[kc + 1];
+    const float idtg = 1.0f / dtg;
     for (int cc = wid; cc < TC; cc += 4) {
         const int col = col0 + cc;
         if (col >= nc) break;  // uniform across the wavefront
@@ -111,20 +110,20 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
             float Fm = 0.0f, Fp = 0.0f, Hm = 0.0f, Hp = 0.0f;
             if (k >= 1) {
                 const float rJf = 0.5f * (rJm + rJ);
-                a0 = dtg * (rJf / c.J) * g33;
+                a0 = dtg * (rJf * c.iJ) * g33;
                 e0 = a0 * (0.5f * (hm + c.h));
                 Fm = rJf * (u3 * g33);
                 Hm = Fm * (0.5f * (hm + c.h));
             }
             if (k + 1 <= nl - 1) {
                 const float rJf = 0.5f * (rJ + rJp);
-                a1 = dtg * (-rJf / c.J) * g33p;
+                a1 = dtg * (-rJf * c.iJ) * g33p;
                 e1 = a1 * (0.5f * (c.h + hp));
                 Fp = rJf * (u3p * g33p);
                 Hp = Fp * (0.5f * (c.h + hp));
             }
-            Rr = dtg * (-(Fp - Fm) / c.J);
-            Re = dtg * (-(Hp - Hm) / c.J);
+            Rr = dtg * (-(Fp - Fm) * c.iJ);
+            Re = dtg * (-(Hp - Hm) * c.iJ);
         }
         // face row k: needs centre k - 1 (lane k - 1) and centre k (own lane)
         CV a;
@@ -153,35 +152,29 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         }
         // parallel cyclic reduction on the 64 face equations, in float64: the
         // acoustic Schur system is stiff (dtg c / dz >> 1) and PCR's
-        // elimination order loses more in float32 than a Thomas sweep
+        // elimination order loses more in float32 than a Thomas sweep.
+        // Step s eliminates the couplings to rows k -+ s with the neighbour
+        // rows divided by their own diagonal: each lane shares
+        // (L, U, r) / D and forms
+        //   L' = -L La~,  U' = -U Ub~,  D' = D - L Ua~ - U Lb~,  r' = r - L ra~ - U rb~.
+        // Row k couples only to rows k -+ s after step s / 2, so for k < s
+        // (k + s > 63) its L (U) is exactly 0, and the out-of-range shuffles,
+        // which return the lane's own finite values, are multiplied by 0.
         double Ld = L, Dd = D, Ud = Up, rd = rhs;
 #pragma unroll
         for (int s = 1; s < 64; s <<= 1) {
-            const bool lo_ok = k >= s, hi_ok = k + s <= 63;
-            double La = pcr::up(Ld, s), Da = pcr::up(Dd, s), Ua = pcr::up(Ud, s), ra = pcr::up(rd, s);
-            double Lb = pcr::dn(Ld, s), Db = pcr::dn(Dd, s), Ub = pcr::dn(Ud, s), rb = pcr::dn(rd, s);
-            if (!lo_ok) {
-                La = 0.0;
-                Da = 1.0;
-                Ua = 0.0;
-                ra = 0.0;
-            }
-            if (!hi_ok) {
-                Lb = 0.0;
-                Db = 1.0;
-                Ub = 0.0;
-                rb = 0.0;
-            }
-            const double al = -Ld / Da, be = -Ud / Db;
-            const double nL = al * La, nU = be * Ub;
-            const double nD = Dd + al * Ua + be * Lb;
-            const double nr = rd + al * ra + be * rb;
-            Ld = nL;
-            Ud = nU;
+            const double iD = ca_drcp(Dd);
+            const double Ln = Ld * iD, Un = Ud * iD, rn = rd * iD;
+            const double La = pcr::up(Ln, s), Ua = pcr::up(Un, s), ra = pcr::up(rn, s);
+            const double Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), rb = pcr::dn(rn, s);
+            const double nD = Dd - Ld * Ua - Ud * Lb;
+            const double nr = rd - Ld * ra - Ud * rb;
+            Ld = -Ld * La;
+            Ud = -Ud * Ub;
             Dd = nD;
             rd = nr;
         }
-        const float x = (float)(rd / Dd);  // du_3 at face k
+        const float x = (float)(rd * ca_drcp(Dd));  // du_3 at face k
         const float x1 = pcr::dn(x, 1);   // du_3 at face k + 1
         // Newton update
         const float dr = a0 * x + a1 * x1 - Rr;
@@ -189,7 +182,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         const float rho_n = rho - dr;
         float rhoe_n = rhoe - de;
         const float u3_n = intF ? u3 - x : 0.0f;
-        float fi_e = -de / dtg;
+        float fi_e = -de * idtg;
         // post-Newton upwind correction at the new state
         if (n.upw != 0) {
             const float u3_np = pcr::dn(u3_n, 1);
@@ -208,19 +201,19 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
                 dF = rJf * upw - rJf * cen;
             }
             const float dF1 = pcr::dn(dF, 1);
-            const float corr = hasC ? -(dF1 - dF) / cn.J : 0.0f;
+            const float corr = hasC ? -(dF1 - dF) * cn.iJ : 0.0f;
             rhoe_n += dtg * corr;
             fi_e += corr;
         }
         if (hasC) {
             tU[0][k][cc] = rho_n;
             tU[3][k][cc] = rhoe_n;
-            tF[0][k][cc] = -dr / dtg;
+            tF[0][k][cc] = -dr * idtg;
             tF[1][k][cc] = fi_e;
         }
         if (isF) {
             tU[4][k][cc] = u3_n;
-            tF[2][k][cc] = intF ? -x / dtg : 0.0f;
+            tF[2][k][cc] = intF ? -x * idtg : 0.0f;
         }
     }
     __syncthreads();

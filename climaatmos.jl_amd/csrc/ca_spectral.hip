@@ -36,32 +36,31 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
     const float *u3 = Y + 4 * nC;
     float rho = Y[idx], u1 = Y[nC + idx], u2 = Y[2 * nC + idx], rhoe = Y[3 * nC + idx];
     float u3a = u3f_at(g, u3, k, col), u3b = u3f_at(g, u3, k + 1, col);
-    float s = g.sc[k], s2 = s * s, is2 = 1.0f / s2, Jv = g.Jvc[k];
-    float J = g.Jh[col] * s2 * Jv;
-    float W = c_W[q.j * 4 + q.i];
+    float s = g.sc[k], s2 = s * s, is2 = g.is2c[k], Jv = g.Jvc[k];
+    const Met m = ca_met(c_W[q.j * 4 + q.i], c_iW[q.j * 4 + q.i], g.Jh[col] * s2 * Jv, g.iJh[col] * g.isJc[k]);
     float ct1, ct2;
     ca_contra(g, col, is2, u1, u2, ct1, ct2);
-    float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]), gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
+    float ga = g.g33f[k], gb = g.g33f[k + 1];
     float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
-    const double e_tot_d = (double)rhoe / (double)rho;
-    const double Td = fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) / n.dcv_d, n.dTmin);
+    const double e_tot_d = (double)rhoe * ca_drcp((double)rho);
+    const double Td = fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) * n.dicv_d, n.dTmin);
     float T = (float)Td;
     float p = (float)((double)rho * n.dR_d * Td);
     float Pi = ca_exner(n, p);
     float cu3 = 0.5f * (u3a + u3b);
     // vector Laplacian of c_u
-    float dv = op_div(q, J, ct1, ct2);
+    float dv = op_div(q, m, ct1, ct2);
     float gd1, gd2;
-    op_wgrad(q, W, dv, gd1, gd2);
-    float cr3 = op_curl3(q, J, u1, u2);
+    op_wgrad(q, m, dv, gd1, gd2);
+    float cr3 = op_curl3(q, m, u1, u2);
     float cr1, cr2;
-    op_curl12(q, J, cu3, cr1, cr2);
+    op_curl12(q, m, cu3, cr1, cr2);
     float cc1, cc2;
     ca_cov(g, col, s2, cr1, cr2, cc1, cc2);
     float cc3 = Jv * Jv * cr3;
-    float w3 = op_wcurl3(q, W, J, cc1, cc2);
+    float w3 = op_wcurl3(q, m, cc1, cc2);
     float w1, w2;
-    op_wcurl12(q, W, J, cc3, w1, w2);
+    op_wcurl12(q, m, cc3, w1, w2);
     float wc1, wc2;
     ca_cov(g, col, s2, w1, w2, wc1, wc2);
     float wc3 = Jv * Jv * w3;
@@ -70,17 +69,17 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
     L[2 * nC + idx] = 0.0f - wc3;
     // scalar Laplacian of the dry static energy deviation
     float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi);
-    float psi = n.cp_d * (T - ca_Tref(n, Pi)) - ca_dPhiref(n, p0, Pi0, p);
+    float psi = n.cp_d * (T - ca_Tref(n, Pi)) - ca_dPhiref(n, p0, ca_rcp(p0), Pi0, p);
     float q1, q2;
     op_grad(q, psi, q1, q2);
     float v1, v2;
     ca_contra(g, col, is2, q1, q2, v1, v2);
-    L[3 * nC + idx] = op_wdiv(q, W, J, v1, v2);
+    L[3 * nC + idx] = op_wdiv(q, m, v1, v2);
 }
 
 // --------------------------------------------------------- explicit kernel
 struct ColState {
-    float rhoJ, Kh, Kv, ct1, ct2, u1, u2, rho;  // K = Kh + Kv (horizontal, vertical parts)
+    float rhoJ, irhoJ, Kh, Kv, ct1, ct2, u1, u2, rho;  // K = Kh + Kv (horizontal, vertical parts)
     float l4;                  // C3 of nabla^4 u (hyperdiffusion)
     float dr, d1, d2, de;      // partial centre tendencies
     float sd;                  // dry static energy (vertical diffusion)
@@ -103,13 +102,16 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
     const int64_t nC = g.nC, nc = g.ncol;
     const float *u3 = Y + 4 * nC;
     constexpr bool sph = SPH;
-    const float W = c_W[q.j * 4 + q.i];
-    const float Jh = g.Jh[col];
+    const float W = c_W[q.j * 4 + q.i], iW = c_iW[q.j * 4 + q.i];
+    const float Jh = g.Jh[col], iJh = g.iJh[col];
     const float f3n = g.f3n[col], f12a = g.f12a[col], f12b = g.f12b[col];
-    float Kh = 0.0f;
-    if (VDIFF) Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
+    float Kf = 0.0f;  // face diffusivity: harmonic mean of the (equal) centre values
+    if (VDIFF) {
+        const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
+        Kf = 1.0f / (0.5f * (1.0f / Kh + 1.0f / Kh));
+    }
     ColState prev, cur;
-    prev.rhoJ = prev.Kh = prev.Kv = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
+    prev.rhoJ = prev.irhoJ = prev.Kh = prev.Kv = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
     prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = 0.0f;
     float Gp1 = 0.0f, Gp2 = 0.0f;      // G at the face below the current centre
     float Fs_prev = 0.0f, Fu0_prev = 0.0f, Fu1_prev = 0.0f, Fu2_prev = 0.0f;  // diffusive fluxes
@@ -123,18 +125,19 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
         if (k < nl) {
             const int64_t idx = (int64_t)k * nc + col;
             float rho = Y[idx], u1 = Y[nC + idx], u2 = Y[2 * nC + idx], rhoe = Y[3 * nC + idx];
-            float s = g.sc[k], s2 = s * s, is2 = 1.0f / s2, Jv = g.Jvc[k];
-            float J = Jh * s2 * Jv;
+            float s = g.sc[k], s2 = s * s, is2 = g.is2c[k], Jv = g.Jvc[k];
+            const Met m = ca_met(W, iW, Jh * s2 * Jv, iJh * g.isJc[k]);
+            const float J = m.J;
             float ct1, ct2;
             ca_contra(g, col, is2, u1, u2, ct1, ct2);
-            float ga = 1.0f / (g.Jvf[k] * g.Jvf[k]), gb = 1.0f / (g.Jvf[k + 1] * g.Jvf[k + 1]);
+            float ga = g.g33f[k], gb = g.g33f[k + 1];
             const float Khor = 0.5f * (u1 * ct1 + u2 * ct2);
             const float Kver = 0.25f * (u3a * ga * u3a + u3b * gb * u3b);
             float K = Khor + Kver;
             // temperature from e_int in float64 (e_tot and Phi are large)
-            const double e_tot_d = (double)rhoe / (double)rho;
+            const double e_tot_d = (double)rhoe * ca_drcp((double)rho);
             const double Td =
-                fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) / n.dcv_d, n.dTmin);
+                fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) * n.dicv_d, n.dTmin);
             float T = (float)Td;
             float h = (float)(e_tot_d + n.dR_d * Td);
             float p = (float)((double)rho * n.dR_d * Td);
@@ -143,22 +146,22 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             if (sph) {
                 // ---- horizontal_dynamics_tendency! (advection.jl:36-91)
                 float m1 = rho * ct1, m2 = rho * ct2;
-                float wd = op_wdiv(q, W, J, m1, m2);
+                float wd = op_wdiv(q, m, m1, m2);
                 dr -= 0.5f * wd + 0.5f * wd;  // split_div(rho u, 1)
                 // split_div(rho u, h) = 1/2 wdiv(rho u h) + 1/2 (h wdiv(rho u) + rho u . grad h),
                 // with h taken relative to the element's first node (exact: linear in constants)
                 float h0 = ca_elem0(h);
                 float hh = h - h0;
-                float wdh = op_wdiv(q, W, J, m1 * hh, m2 * hh);
+                float wdh = op_wdiv(q, m, m1 * hh, m2 * hh);
                 float gh1, gh2;
                 op_grad(q, hh, gh1, gh2);
                 de -= 0.5f * wdh + 0.5f * (hh * wd + (m1 * gh1 + m2 * gh2)) + h0 * wd;
                 // u_h -= C12(grad(K + Phi - Phi_r) + cp_d (th' grad Pi + grad(th' Pi) - Pi grad th') / 2)
-                float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi);
-                float A = K - ca_dPhiref(n, p0, Pi0, p);
-                float dPi = ca_dExner(n, p0, Pi0, p);
+                float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi), ip0 = ca_rcp(p0);
+                float A = K - ca_dPhiref(n, p0, ip0, Pi0, p);
+                float dPi = ca_dExner(n, p0, ip0, Pi0, p);
                 float thPi = T - ca_Tref(n, Pi);
-                float th = thPi / Pi;
+                float th = thPi * ca_rcp(Pi);
                 float g1, g2, a1, a2, b1, b2, c1, c2;
                 op_grad(q, A, g1, g2);
                 op_grad(q, dPi, a1, a2);
@@ -171,18 +174,18 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
                     float v1, v2;
                     ca_contra(g, col, is2, X1, X2, v1, v2);
-                    float dv = op_div(q, J, v1, v2);
+                    float dv = op_div(q, m, v1, v2);
                     float gd1, gd2;
-                    op_wgrad(q, W, dv, gd1, gd2);
-                    float cr3 = op_curl3(q, J, X1, X2);
+                    op_wgrad(q, m, dv, gd1, gd2);
+                    float cr3 = op_curl3(q, m, X1, X2);
                     float cr1, cr2;
-                    op_curl12(q, J, X3, cr1, cr2);
+                    op_curl12(q, m, X3, cr1, cr2);
                     float cc1, cc2;
                     ca_cov(g, col, s2, cr1, cr2, cc1, cc2);
                     float cc3 = Jv * Jv * cr3;
-                    float w3 = op_wcurl3(q, W, J, cc1, cc2);
+                    float w3 = op_wcurl3(q, m, cc1, cc2);
                     float w1, w2;
-                    op_wcurl12(q, W, J, cc3, w1, w2);
+                    op_wcurl12(q, m, cc3, w1, w2);
                     float wc1, wc2;
                     ca_cov(g, col, s2, w1, w2, wc1, wc2);
                     float wc3 = Jv * Jv * w3;
@@ -193,12 +196,12 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     op_grad(q, Ls, s1, s2g);
                     float r1, r2;
                     ca_contra(g, col, is2, rho * s1, rho * s2g, r1, r2);
-                    de -= n.nu_s * op_wdiv(q, W, J, r1, r2);
+                    de -= n.nu_s * op_wdiv(q, m, r1, r2);
                 }
             }
             // ---- explicit_vertical_advection_tendency! centre part (advection.jl:273-277)
-            float om3 = sph ? op_wcurl3(q, W, J, u1, u2) : 0.0f;
-            float a3 = f3n / Jv + om3;
+            float om3 = sph ? op_wcurl3(q, m, u1, u2) : 0.0f;
+            float a3 = f3n * g.iJvc[k] + om3;
             d1 -= J * (0.0f - a3 * ct2);
             d2 -= J * (a3 * ct1 - 0.0f);
             // ---- Rayleigh sponge on u_h (rayleigh_sponge.jl:39-44)
@@ -208,6 +211,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 d2 += -beta * u2;
             }
             cur.rhoJ = rho * J;
+            cur.irhoJ = ca_rcp(cur.rhoJ);
             cur.Kh = Khor;
             cur.Kv = Kver;
             cur.ct1 = ct1;
@@ -224,7 +228,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 cur.sd = n.cp_d * (T - n.T_0) + g.Phic[k];
                 if (sph) {
                     const float *B = g.basis;
-                    float is = 1.0f / s;
+                    float is = ca_rcp(s);
                     cur.uc0 = (u1 * B[0 * nc + col] + u2 * B[3 * nc + col]) * is;
                     cur.uc1 = (u1 * B[1 * nc + col] + u2 * B[4 * nc + col]) * is;
                     cur.uc2 = (u1 * B[2 * nc + col] + u2 * B[5 * nc + col]) * is;
@@ -242,13 +246,16 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             // the horizontal weak curl of u_3 at face k needs every lane of the element
             float x = 0.0f, y = 0.0f;
             const bool interior = (k >= 1 && k <= nl - 1);
-            float sf = g.sf[k], Jf = Jh * sf * sf * g.Jvf[k];
-            if (sph && interior) op_wcurl12(q, W, Jf, u3a, x, y);
+            const float sf = g.sf[k];
+            const Met mf = ca_met(W, iW, Jh * sf * sf * g.Jvf[k], iJh * g.isJf[k]);
+            const float Jf = mf.J, iJf = mf.iJ;
+            if (sph && interior) op_wcurl12(q, mf, u3a, x, y);
             if (interior) {
-                float fu3 = u3a / (g.Jvf[k] * g.Jvf[k]);
-                float wa = -(cur.u2 - prev.u2) / Jf + x;
-                float wb = (cur.u1 - prev.u1) / Jf + y;
-                float A1 = f12a / sf + wa, A2 = f12b / sf + wb;
+                float fu3 = u3a * g.g33f[k];
+                float wa = -(cur.u2 - prev.u2) * iJf + x;
+                float wb = (cur.u1 - prev.u1) * iJf + y;
+                const float isf = g.isf[k];
+                float A1 = f12a * isf + wa, A2 = f12b * isf + wb;
                 float rJf = 0.5f * (prev.rhoJ + cur.rhoJ);
                 float b3 = rJf * fu3;
                 G1 = Jf * (A2 * b3);
@@ -259,12 +266,12 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 du3 = -(Jf * (A1 * i2 - A2 * i1) + ((cur.Kh - prev.Kh) + (cur.Kv - prev.Kv)));
                 if (SPH && HYPER) {
                     float wA = prev.rhoJ, wB = cur.rhoJ;
-                    du3 -= n.nu_v * ((0.5f * (wA * prev.l4 + wB * cur.l4)) / (0.5f * (wA + wB)));
+                    du3 -= n.nu_v * ((0.5f * (wA * prev.l4 + wB * cur.l4)) * ca_rcp(0.5f * (wA + wB)));
                 }
                 if (VDIFF) {
                     // vertical_diffusion_boundary_layer.jl:90-103 with K_h = D0 (H = Inf)
-                    float rK = (0.5f * (prev.rho + cur.rho)) / (0.5f * (1.0f / Kh + 1.0f / Kh));
-                    float iJv = 1.0f / g.Jvf[k];
+                    float rK = (0.5f * (prev.rho + cur.rho)) * Kf;
+                    float iJv = g.iJvf[k];
                     Fs = Jf * (iJv * iJv * (-(rK * (cur.sd - prev.sd))));
                     float fac = Jf * iJv * (-2.0f * rK) * 0.5f;
                     Fu0 = fac * (cur.uc0 - prev.uc0) * iJv;
@@ -278,14 +285,14 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
         if (k - 1 >= k0 && k - 1 < k1) {
             const int kk = k - 1;
             const int64_t idx = (int64_t)kk * nc + col;
-            float d1 = prev.d1 - (0.5f * (Gp1 + G1)) / prev.rhoJ;
-            float d2 = prev.d2 - (0.5f * (Gp2 + G2)) / prev.rhoJ;
+            float d1 = prev.d1 - (0.5f * (Gp1 + G1)) * prev.irhoJ;
+            float d2 = prev.d2 - (0.5f * (Gp2 + G2)) * prev.irhoJ;
             float de = prev.de;
             if (VDIFF) {
                 float s = g.sc[kk];
-                float Jc = Jh * s * s * g.Jvc[kk];
-                de -= (Fs - Fs_prev) / Jc;
-                float e0 = (Fu0 - Fu0_prev) / Jc, e1 = (Fu1 - Fu1_prev) / Jc, e2 = (Fu2 - Fu2_prev) / Jc;
+                const float iJc = iJh * g.isJc[kk];
+                de -= (Fs - Fs_prev) * iJc;
+                float e0 = (Fu0 - Fu0_prev) * iJc, e1 = (Fu1 - Fu1_prev) * iJc, e2 = (Fu2 - Fu2_prev) * iJc;
                 float c1, c2;
                 if (sph) {
                     const float *B = g.basis;
@@ -295,8 +302,9 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     c1 = e0;
                     c2 = e1;
                 }
-                d1 -= c1 / prev.rho;
-                d2 -= c2 / prev.rho;
+                const float irho = ca_rcp(prev.rho);
+                d1 -= c1 * irho;
+                d2 -= c2 * irho;
             }
             Yt[idx] = prev.dr;
             Yt[nC + idx] = d1;

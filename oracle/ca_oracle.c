@@ -58,6 +58,7 @@ struct ora_ctx {
     real *t[12];
     real *FE[4], *FI[4], *Ut, *U, *Rv, *dY;
     double dtgamma;
+    int round_storage;     /* ora_set_storage_rounding */
 };
 
 #define IDX(k, col) ((int64_t)(k) * c->ncol + (col))
@@ -1128,6 +1129,18 @@ static ars343_t ars343(void) {
     return t;
 }
 
+/* Storage rounding: with it on, ora_step rounds every array a float32
+ * implementation stores (each stage state, explicit and implicit tendency,
+ * and the new state) to float32, and computes in the build's precision
+ * between the roundings.  It measures the error float32 storage alone
+ * causes (the float64 build only; no effect in the float32 build). */
+void ora_set_storage_rounding(ora_ctx *c, int32_t on) { c->round_storage = on; }
+static void round_storage(const ora_ctx *c, real *x, int64_t n) {
+    if (!c->round_storage) return;
+    ORA_PAR_FOR
+    for (int64_t q = 0; q < n; ++q) x[q] = (real)(float)x[q];
+}
+
 void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
     real *u = (real *)Yd;
     ars343_t tab = ars343();
@@ -1136,6 +1149,7 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
         /* stage 1 (explicit): U1 = u */
         set_precomputed(c, u);
         remaining_tendency(c, c->FE[0], u);
+        round_storage(c, c->FE[0], S);
         for (int i = 1; i < 4; ++i) {
             real *Ut = c->Ut, *U = c->U;
             ORA_PAR_FOR
@@ -1148,6 +1162,7 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
                 Ut[q] = u[q] + (real)dt * s;
             }
             dss_state(c, Ut);
+            round_storage(c, Ut, S);
             real dtg = (real)(dt * tab.aI[i][i]);
             memcpy(U, Ut, sizeof(real) * S);
             /* Newton iteration: cache_imp!, T_imp!, Wfact, ldiv! */
@@ -1166,11 +1181,14 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
                 ORA_PAR_FOR
                 for (int64_t q = 0; q < S; ++q) U[q] += dtg * c->dY[q];
             }
+            round_storage(c, U, S);
             ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) c->FI[i][q] = (U[q] - Ut[q]) / dtg;
+            round_storage(c, c->FI[i], S);
             /* end of stage: cache! then T_exp! */
             set_precomputed(c, U);
             remaining_tendency(c, c->FE[i], U);
+            round_storage(c, c->FE[i], S);
         }
         ORA_PAR_FOR
         for (int64_t q = 0; q < S; ++q) {
@@ -1180,6 +1198,7 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
             u[q] = u[q] + (real)dt * s;
         }
         dss_state(c, u);
+        round_storage(c, u, S);
         set_precomputed(c, u);
     }
 }

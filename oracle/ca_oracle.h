@@ -71,6 +71,10 @@ void ora_ldiv(ora_ctx *c, ora_real *dY, const ora_real *R);
 void ora_correct_implicit(ora_ctx *c, ora_real *Yt, ora_real *Y);
 void ora_dss(ora_ctx *c, ora_real *Y);
 void ora_step(ora_ctx *c, ora_real *Y, double dt, int32_t nsteps);
+/* round every array a float32 implementation stores to float32 inside
+ * ora_step (stage states, tendencies, new state): measures the error of
+ * float32 storage alone */
+void ora_set_storage_rounding(ora_ctx *c, int32_t on);
 /* the ARS343 tableau used by ora_step (4x4 row-major aE, aI; b; c) */
 void ora_tableau(double *aE, double *aI, double *b, double *c);
 

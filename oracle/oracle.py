@@ -82,6 +82,7 @@ class Oracle:
         lib.ora_update_jacobian.argtypes = [C.c_void_p, rp, C.c_double]
         lib.ora_ldiv.argtypes = [C.c_void_p, rp, rp]
         lib.ora_step.argtypes = [C.c_void_p, rp, C.c_double, C.c_int32]
+        lib.ora_set_storage_rounding.argtypes = [C.c_void_p, C.c_int32]
         d = C.POINTER(C.c_double)
         lib.ora_get_precomputed.argtypes = [C.c_void_p, d, d, d, d]
         lib.ora_get_laplacians.argtypes = [C.c_void_p, d]
@@ -193,6 +194,11 @@ class Oracle:
 
     def step(self, Y, dt, nsteps=1):
         self.lib.ora_step(self.h, self._v(Y), float(dt), int(nsteps))
+
+    def set_storage_rounding(self, on=True):
+        """Round stage states, tendencies and the new state to float32 inside
+        step(): the error of float32 storage alone (float64 build)."""
+        self.lib.ora_set_storage_rounding(self.h, 1 if on else 0)
 
     def precomputed(self):
         nC = self.grid.nlev * self.grid.ncol

@@ -2,18 +2,18 @@
 the same float32-rounded seeded inputs, through the C ABI.
 
 Gates (relative L2 per field, native float32 vs oracle float64):
-  * hooks (one evaluation)            <= 1e-5  (u_3 tendency: <= 1e-4, see below)
-  * one step                          <= 1e-5  for rho, u_1, u_2, rho e_tot
-  * 100 column steps                  <= 1e-3  for rho, u_1, u_2, rho e_tot
-The u_3 field starts at rest, so its one-step value is the increment of a
-small difference of large hydrostatic terms; its gate is stated separately.
-After 100 column steps u_3 (covariant, w times the ~1 km face spacing) is an
-acoustic oscillation of w ~ 0.05 m/s whose error is set by storing the
-state in float32: the float64 oracle with its state rounded to float32 once
-per step already departs from the unrounded oracle by ~4.5e-4, and the
-oracle built in float32 throughout by ~6e-3.  The gate is three times the
-once-per-step rounding floor, measured in the test, and half the
-float32-throughout error.
+  * hooks (one evaluation)             <= 1e-5  (u_3 tendency: <= 1e-4)
+  * one step, rho, u_1, u_2, rho e_tot <= 1e-5
+  * 100 column steps, the same fields  <= 1e-3
+  * u_3, against its float32 storage floor: the error of the float64 oracle
+    run with every array a float32 implementation stores (stage states,
+    tendencies, new state) rounded to float32.  u_3 starts at rest, so its
+    value after a step is a small increment of a small difference of large
+    hydrostatic terms, and storage rounding alone gives it ~1e-4 after one
+    step and ~3e-3 after 100 column steps.  The native u_3 must stay within
+    3x that floor after one step and 2x over a few steps, at or below it
+    after 100 column steps, and below half the error of the oracle built in
+    float32 throughout.
 At the full workload sizes the oracle is too slow for many steps, so those
 tests check size-independent properties: finiteness, mass conservation to
 float32 round-off, and bitwise continuity of duplicated element-edge columns.
@@ -105,6 +105,20 @@ def _run_steps(grid, num, Y0, dt, nsteps):
     return got, Yo, Y32
 
 
+def _storage_floor(grid, num, Y32, dt, nsteps, ref):
+    """Per-field error against ``ref`` of the float64 oracle stepping with
+    float32 storage rounding."""
+    from oracle import Oracle
+
+    o = Oracle(grid, PAR, num)
+    Y = Y32.copy()
+    o.dss(Y)
+    o.set_precomputed(Y)
+    o.set_storage_rounding(True)
+    o.step(Y, dt, nsteps)
+    return field_errors(Y, ref, grid)
+
+
 def _column():
     zf = G.tanh_stretched_faces(30, 30000.0, 500.0)
     g = G.columns(1, zf)
@@ -127,61 +141,54 @@ def test_hooks_match_oracle(case):
             assert e <= tol, (hook, name, e)
 
 
-def _float32_floors(g, num, Y32, dt, nsteps):
-    """u_3 errors of (a) the float64 oracle with its state rounded to float32
-    after every step and (b) the oracle built in float32, both against the
-    unrounded float64 oracle."""
+def _float32_throughout(g, num, Y32, dt, nsteps, ref):
+    """u_3 error against ``ref`` of the oracle built in float32."""
     from oracle import Oracle
 
-    def start(f32=False):
-        o = Oracle(g, PAR, num, f32=f32)
-        Y = Y32.astype(o.dtype).copy()
-        o.dss(Y)
-        o.set_precomputed(Y)
-        return o, Y
-
-    o, ref = start()
-    o.step(ref, dt, nsteps)
-    o, Yq = start()
-    for _ in range(nsteps):
-        o.step(Yq, dt, 1)
-        Yq[:] = Yq.astype(np.float32).astype(np.float64)
-    o, Yf = start(f32=True)
-    o.step(Yf, dt, nsteps)
-    return (field_errors(Yq, ref, g)["u3"], field_errors(Yf.astype(np.float64), ref, g)["u3"])
+    o = Oracle(g, PAR, num, f32=True)
+    Y = Y32.astype(np.float32)
+    o.dss(Y)
+    o.set_precomputed(Y)
+    o.step(Y, dt, nsteps)
+    return field_errors(Y.astype(np.float64), ref, g)["u3"]
 
 
 def test_column_one_step_and_hundred_steps():
     g, num, Y0 = _column()
-    got, ref, _ = _run_steps(g, num, Y0, 10.0, 1)
+    got, ref, Y32 = _run_steps(g, num, Y0, 10.0, 1)
     e1 = field_errors(got, ref, g)
-    print("1 step", e1)
+    fl1 = _storage_floor(g, num, Y32, 10.0, 1, ref)
+    print("1 step", e1, "u3 storage floor", fl1["u3"])
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e1[name] <= 1e-5
-    assert e1["u3"] <= 1e-4
+    assert e1["u3"] <= 3.0 * fl1["u3"]
     got, ref, Y32 = _run_steps(g, num, Y0, 10.0, 100)
     e100 = field_errors(got, ref, g)
-    round_floor, f32_err = _float32_floors(g, num, Y32, 10.0, 100)
-    print("100 steps", e100, "u3 floors: rounded-per-step", round_floor, "float32 throughout", f32_err)
+    fl = _storage_floor(g, num, Y32, 10.0, 100, ref)["u3"]
+    f32_err = _float32_throughout(g, num, Y32, 10.0, 100, ref)
+    print("100 steps", e100, "u3 storage floor", fl, "float32 throughout", f32_err)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e100[name] <= 1e-3, (name, e100[name])
-    assert e100["u3"] <= 3.0 * round_floor, (e100["u3"], round_floor)
+    assert e100["u3"] <= fl, (e100["u3"], fl)
     assert e100["u3"] <= 0.5 * f32_err, (e100["u3"], f32_err)
 
 
 def test_sphere_one_and_five_steps():
     g, num, Y0 = _sphere_small()
-    got, ref, _ = _run_steps(g, num, Y0, 100.0, 1)
+    got, ref, Y32 = _run_steps(g, num, Y0, 100.0, 1)
     e1 = field_errors(got, ref, g)
-    print("1 step", e1)
+    fl1 = _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
+    print("1 step", e1, "u3 storage floor", fl1)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e1[name] <= 1e-5
-    assert e1["u3"] <= 1e-4
-    got, ref, _ = _run_steps(g, num, Y0, 100.0, 5)
+    assert e1["u3"] <= 3.0 * fl1
+    got, ref, Y32 = _run_steps(g, num, Y0, 100.0, 5)
     e5 = field_errors(got, ref, g)
-    print("5 steps", e5)
-    for name, e in e5.items():
-        assert e <= 1e-3, (name, e)
+    fl5 = _storage_floor(g, num, Y32, 100.0, 5, ref)["u3"]
+    print("5 steps", e5, "u3 storage floor", fl5)
+    for name in ("rho", "u1", "u2", "rhoe"):
+        assert e5[name] <= 1e-3, (name, e5[name])
+    assert e5["u3"] <= 2.0 * fl5
 
 
 @pytest.mark.parametrize("dss_chunk", [1, 2, 3, 4, 10])
@@ -209,10 +216,11 @@ def test_small_sphere_dss_level_chunks(dss_chunk):
     o.set_precomputed(ref)
     o.step(ref, 100.0, 1)
     e = field_errors(got, ref, g)
-    print(dss_chunk, e)
+    fl = _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
+    print(dss_chunk, e, "u3 storage floor", fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e[name] <= 1e-5, (name, e[name])
-    assert e["u3"] <= 1e-4
+    assert e["u3"] <= 3.0 * fl
 
 
 def test_first_order_upwinding_and_rayleigh_sponge_column():
@@ -234,12 +242,13 @@ def test_c1_sized_sphere_one_step_parity():
     g = G.cubed_sphere(16, zf, PAR.planet_radius, PAR.Omega)
     num = P.Numerics(dt=100.0)
     Y0 = I.sphere_state(g, PAR, seed=1)
-    got, ref, _ = _run_steps(g, num, Y0, 100.0, 1)
+    got, ref, Y32 = _run_steps(g, num, Y0, 100.0, 1)
     e = field_errors(got, ref, g)
-    print(e)
+    fl = _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
+    print(e, "u3 storage floor", fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e[name] <= 1e-5, (name, e[name])
-    assert e["u3"] <= 1e-3
+    assert e["u3"] <= 3.0 * fl
 
 
 def test_full_size_properties_h30():

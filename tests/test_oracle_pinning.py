@@ -604,3 +604,31 @@ def test_correction_plus_central_equals_upwind_and_vanleer_is_bounded():
                 lo, hi = min(h[f - 1], h[f]), max(h[f - 1], h[f])
                 span = hi - lo
                 assert lo - 1e-9 * abs(hi) - 1e-12 * span <= face <= hi + 1e-9 * abs(hi)
+
+
+def test_storage_rounding_switch():
+    """Off by default (stepping twice from one state is bitwise reproducible
+    and leaves values that float32 cannot hold); on, the stepped state is
+    float32-representable and differs from the unrounded one at round-off
+    level."""
+    zf = G.tanh_stretched_faces(12, 30000.0, 500.0)
+    g = G.columns(2, zf)
+    num = P.Numerics(hyperdiff=False, dt=10.0)
+    Y0 = I.column_state(g, PAR, seed=3)
+
+    def run(rnd):
+        o = Oracle(g, PAR, num)
+        Y = Y0.copy()
+        o.dss(Y)
+        o.set_precomputed(Y)
+        if rnd:
+            o.set_storage_rounding(True)
+        o.step(Y, 10.0, 3)
+        return Y
+
+    a, a2, b = run(False), run(False), run(True)
+    np.testing.assert_array_equal(a, a2)
+    assert np.any(a != a.astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(b, b.astype(np.float32).astype(np.float64))
+    rel = np.linalg.norm(a - b) / np.linalg.norm(a)
+    assert 0 < rel < 1e-5

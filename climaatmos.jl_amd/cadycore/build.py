@@ -12,6 +12,8 @@ PKG = os.path.dirname(_HERE)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 LIB = os.path.join(_HERE, "libcadycore.so")
+HOST_LIB = os.path.join(_HERE, "libcadycore_host.so")
+HOST_SOURCES = ["ca_dss_table.cpp", "ca_host_tables.cpp"]
 SOURCES = ["ca_unity.hip"]
 ARCH = os.environ.get("CA_OFFLOAD_ARCH", "gfx950")
 
@@ -32,15 +34,26 @@ def command(out=LIB, extra=()):
     ]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = command(tmp)
+def host_command(out=HOST_LIB):
+    """The host-only table library the CPU tests load (g++, no HIP)."""
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
+    return ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", CSRC, "-o", out, *srcs]
+
+
+def _run(cmd, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [tmp if c == out else c for c in cmd]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"{cmd[0]} failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    _run(command(LIB), LIB, verbose)
+    _run(host_command(HOST_LIB), HOST_LIB, verbose)
     return LIB
 
 

@@ -68,7 +68,7 @@ EXPORTED = (
     "ca_check_finite", "ca_profile_enable", "ca_profile_count", "ca_profile_get",
     "ca_set_tuning", "ca_get_tuning",
 )
-CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK = 0, 1, 2
+CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND = 0, 1, 2, 3
 
 
 def load_library(path: str = LIB_PATH):

@@ -1,7 +1,9 @@
 // ca_api.hip -- host side of the C ABI declared in include/cadycore.h:
 // handle life cycle, geometry upload, the hook-level entry points and the
 // fused ARS343 step driver.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -9,7 +11,11 @@
 
 #include "../../include/cadycore.h"
 #include "ca_kernels.h"
+#include "ca_dss_table.h"
 
+
+// message of the last failed ca_create (no handle exists to hold it)
+static std::string g_create_error = "no error";
 
 namespace {
 
@@ -56,7 +62,8 @@ struct ca_handle {
     double aE[4][4], aI[4][4], b[4];
     int newton_variant = 1;
     int explicit_chunk = 16;
-    int dss_chunk = 2;
+    int dss_chunk = 8;
+    int dss_kind = 0;  // 0: LDS-staged per element cluster, 1: direct gather
 };
 
 namespace {
@@ -163,15 +170,32 @@ ca_status filter_columns(ca_handle *h, const float *src, float *dst, hipStream_t
     return check_launch(h, "filter");
 }
 
-// gather DSS of a full state-shaped buffer, src -> dst (out of place)
-ca_status dss_gather_state(ca_handle *h, const float *src, float *dst, int filter, hipStream_t st, int kid) {
+// grid of the LDS-staged DSS: (cluster, level chunk) pairs, padded to a
+// multiple of 8 for the XCD remap
+inline unsigned dss_lds_grid(const DevGeom &g, int chunk) {
+    const unsigned nb = (unsigned)g.ncl * cdiv(g.nlev, chunk);
+    return 8 * cdiv(nb, 8);
+}
+
+// DSS of NC centre planes (vector pair at VEC) and, with FACE, a face plane,
+// src -> dst: the LDS-staged kernel on the sphere
+template <int NC, int VEC, bool FACE>
+void launch_dss(const ca_handle *h, const float *src, float *dst, int filter, hipStream_t st) {
     const DevGeom &g = h->g;
     const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
+    if (h->dss_kind == 0 && g.ncl > 0)
+        k_dss_lds<NC, VEC, FACE><<<dss_lds_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+    else
+        k_dss_gather<NC, VEC, FACE><<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+}
+
+// DSS of a full state-shaped buffer, src -> dst (out of place)
+ca_status dss_gather_state(ca_handle *h, const float *src, float *dst, int filter, hipStream_t st, int kid) {
     {
         Launch l(h, st, kid);
-        k_dss_gather<4, 1, true><<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+        launch_dss<4, 1, true>(h, src, dst, filter, st);
     }
-    return check_launch(h, "k_dss_gather(state)");
+    return check_launch(h, "k_dss(state)");
 }
 
 // in-place state DSS (dss! hook): copy to the scratch buffer, gather back
@@ -196,8 +220,7 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
         }
         {
             Launch l(h, st, KID_DSS_LAPLACIAN);
-            const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
-            k_dss_gather<4, 0, false><<<dss_grid(g, ch), 256, 0, st>>>(g, h->raw, h->L, 0, ch);
+            launch_dss<4, 0, false>(h, h->raw, h->L, 0, st);
         }
     }
     float *out = (dss_out && dss) ? h->raw : Yt;
@@ -263,19 +286,10 @@ ca_status ensure_bands(ca_handle *h) {
     return CA_OK;
 }
 
-}  // namespace
 
-extern "C" {
-
-ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_config *cfg, ca_handle **out) {
-    if (!geom || !par || !cfg || !out) return CA_ERR_ARG;
-    *out = nullptr;
+// ca_create's body: fills h; on failure the caller frees h
+ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_config *cfg, ca_handle *h) {
     const int nl = geom->nlev, nc = geom->ncol;
-    if (nl < 3 || nc < 1) return CA_ERR_ARG;
-    if (geom->nelem > 0 && nc != 16 * geom->nelem) return CA_ERR_ARG;
-    if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3) return CA_ERR_ARG;
-    if (cfg->vert_diff && !cfg->D0_col) return CA_ERR_ARG;
-    ca_handle *h = new ca_handle();
     HIPCHK(h, hipGetDevice(&h->device));
     DevGeom &g = h->g;
     g.nelem = geom->nelem;
@@ -387,61 +401,38 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
         if ((s = upload(h, &p, tofloat(cfg->D0_col, nc)))) return s;
         g.D0_col = p;
     }
-    // DSS gather table: a row per column, the copies of its node in the
-    // group's canonical order, padded to four slots with (own column, 0)
+    // DSS tables (gather rows per column; element clusters on the sphere)
     {
-        const int ngrp = geom->nelem > 0 ? geom->n_dss : 0;
-        std::vector<int> cgrp(nc, -1);
-        for (int gi = 0; gi < ngrp; ++gi)
-            for (int t = geom->dss_ptr[gi]; t < geom->dss_ptr[gi + 1]; ++t) {
-                const int c = geom->dss_col[t];
-                if (c < 0 || c >= nc || cgrp[c] != -1 || geom->dss_ptr[gi + 1] - geom->dss_ptr[gi] > 4) {
-                    delete h;
-                    return CA_ERR_ARG;
-                }
-                cgrp[c] = gi;
-            }
-        std::vector<int> tc(4 * (size_t)nc);
-        std::vector<float> tw(4 * (size_t)nc, 0.0f), tM(16 * (size_t)nc, 0.0f);
-        const double *Bs = geom->basis;
-        auto bas = [&](int plane, int c) { return Bs[(size_t)plane * nc + c]; };
-        for (int c = 0; c < nc; ++c) {
-            for (int t = 0; t < 4; ++t) tc[4 * (size_t)c + t] = c;
-            const int gi = cgrp[c];
-            if (gi < 0) {
-                tw[4 * (size_t)c] = 1.0f;
-                tM[16 * (size_t)c + 0] = 1.0f;
-                tM[16 * (size_t)c + 3] = 1.0f;
-                continue;
-            }
-            const int b0 = geom->dss_ptr[gi], m = geom->dss_ptr[gi + 1] - b0;
-            for (int t = 0; t < m; ++t) {
-                const int mcol = geom->dss_col[b0 + t];
-                const double w = geom->dss_w[b0 + t];
-                tc[4 * (size_t)c + t] = mcol;
-                tw[4 * (size_t)c + t] = (float)w;
-                double m00 = 0, m01 = 0, m10 = 0, m11 = 0;
-                for (int d = 0; d < 3; ++d) {
-                    m00 += bas(6 + d, c) * bas(d, mcol);
-                    m01 += bas(6 + d, c) * bas(3 + d, mcol);
-                    m10 += bas(9 + d, c) * bas(d, mcol);
-                    m11 += bas(9 + d, c) * bas(3 + d, mcol);
-                }
-                float *Mt = &tM[16 * (size_t)c + 4 * t];
-                Mt[0] = (float)(w * m00);
-                Mt[1] = (float)(w * m01);
-                Mt[2] = (float)(w * m10);
-                Mt[3] = (float)(w * m11);
-            }
-        }
+        DssTable T;
+        const std::string msg = ca_build_dss_table(geom->nelem, nc, geom->nelem > 0 ? geom->n_dss : 0, geom->dss_ptr,
+                                                   geom->dss_col, geom->dss_w, geom->basis, T);
+        if (!msg.empty()) return fail(h, CA_ERR_ARG, "ca_create: " + msg);
         int *d_tc = nullptr;
         float *d_tw = nullptr, *d_tM = nullptr;
-        if ((s = upload(h, &d_tc, tc))) return s;
-        if ((s = upload(h, &d_tw, tw))) return s;
-        if ((s = upload(h, &d_tM, tM))) return s;
+        if ((s = upload(h, &d_tc, T.tc))) return s;
+        if ((s = upload(h, &d_tw, T.tw))) return s;
+        if ((s = upload(h, &d_tM, T.tM))) return s;
         g.gtc = reinterpret_cast<const int4 *>(d_tc);
         g.gtw = reinterpret_cast<const float4 *>(d_tw);
         g.gtM = reinterpret_cast<const float4 *>(d_tM);
+        g.ncl = (int)T.nE.size();
+        if (g.ncl > 0) {
+            int *d_list = nullptr, *d_nE = nullptr, *d_ccol = nullptr;
+            uint16_t *d_slot = nullptr;
+            float *d_cw = nullptr, *d_cM = nullptr;
+            if ((s = upload(h, &d_list, T.list))) return s;
+            if ((s = upload(h, &d_nE, T.nE))) return s;
+            if ((s = upload(h, &d_ccol, T.ccol))) return s;
+            if ((s = upload(h, &d_slot, T.slot))) return s;
+            if ((s = upload(h, &d_cw, T.cw))) return s;
+            if ((s = upload(h, &d_cM, T.cM))) return s;
+            g.dl_list = d_list;
+            g.dl_nE = d_nE;
+            g.dl_col = d_ccol;
+            g.dl_slot = reinterpret_cast<const ushort4 *>(d_slot);
+            g.dl_w = reinterpret_cast<const float4 *>(d_cw);
+            g.dl_M = reinterpret_cast<const float4 *>(d_cM);
+        }
     }
     float W[16], iW[16], Dm[4][4];
     for (int j = 0; j < 4; ++j)
@@ -501,6 +492,28 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     HIPCHK(h, hipMemset(h->L, 0, 4 * (size_t)g.nC * sizeof(float)));
     ars343(h);
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
+    return CA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_config *cfg, ca_handle **out) {
+    if (!geom || !par || !cfg || !out) return CA_ERR_ARG;
+    *out = nullptr;
+    const int nl = geom->nlev, nc = geom->ncol;
+    if (nl < 3 || nc < 1) return CA_ERR_ARG;
+    if (geom->nelem > 0 && nc != 16 * geom->nelem) return CA_ERR_ARG;
+    if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3) return CA_ERR_ARG;
+    if (cfg->vert_diff && !cfg->D0_col) return CA_ERR_ARG;
+    ca_handle *h = new ca_handle();
+    const ca_status s = create_impl(geom, par, cfg, h);
+    if (s != CA_OK) {
+        g_create_error = h->err.empty() ? "ca_create: invalid argument" : h->err;
+        ca_destroy(h);
+        return s;
+    }
     *out = h;
     return CA_OK;
 }
@@ -518,7 +531,7 @@ ca_status ca_destroy(ca_handle *h) {
 
 int64_t ca_state_size(const ca_handle *h) { return h ? h->S : 0; }
 
-const char *ca_last_error(const ca_handle *h) { return h ? h->err.c_str() : "null handle"; }
+const char *ca_last_error(const ca_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
 ca_status ca_set_precomputed(ca_handle *h, float *Y, void *stream) {
     if (!h || !Y) return CA_ERR_ARG;
@@ -767,6 +780,11 @@ ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value) {
         h->dss_chunk = value;
         return CA_OK;
     }
+    if (knob == CA_TUNE_DSS_KIND) {
+        if (value != 0 && value != 1) return fail(h, CA_ERR_ARG, "DSS kind must be 0 (LDS-staged) or 1 (direct)");
+        h->dss_kind = value;
+        return CA_OK;
+    }
     return fail(h, CA_ERR_ARG, "unknown tuning knob");
 }
 
@@ -775,6 +793,7 @@ ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value) {
     if (knob == CA_TUNE_NEWTON) *value = h->newton_variant;
     else if (knob == CA_TUNE_EXPLICIT_CHUNK) *value = h->explicit_chunk;
     else if (knob == CA_TUNE_DSS_CHUNK) *value = h->dss_chunk;
+    else if (knob == CA_TUNE_DSS_KIND) *value = h->dss_kind;
     else return CA_ERR_ARG;
     return CA_OK;
 }

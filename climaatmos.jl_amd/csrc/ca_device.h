@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ca_dss_table.h"
+
 #define CA_NQ 4
 #define CA_NN 16
 
@@ -45,6 +47,16 @@ struct DevGeom {
     // An interior node is its own single slot of weight 1.
     const int4 *gtc;
     const float4 *gtw, *gtM;
+    // LDS-staged form of the same table.  Elements are grouped into clusters
+    // of at most CA_DSS_OWN elements; a cluster's element list holds its own
+    // elements followed by every other element one of their nodes' copies
+    // lives in (at most CA_DSS_EMAX in all).  Per (cluster, thread): the own
+    // column (-1: none), the four slots as indices into the cluster's
+    // staged values (list position * 16 + node), the weights and transforms.
+    int ncl;
+    const int *dl_list, *dl_nE, *dl_col;
+    const ushort4 *dl_slot;
+    const float4 *dl_w, *dl_M;
     const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
 };
 

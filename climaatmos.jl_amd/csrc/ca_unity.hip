@@ -4,4 +4,5 @@
 #include "ca_newton_pcr.hip"
 #include "ca_spectral.hip"
 #include "ca_dss.hip"
+#include "ca_dss_table.cpp"
 #include "ca_api.hip"

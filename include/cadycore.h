@@ -103,6 +103,8 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *params, const ca_c
                     ca_handle **out);
 ca_status ca_destroy(ca_handle *h);
 int64_t ca_state_size(const ca_handle *h);
+/* message of the last failure on h; with h = NULL, of the last failed
+ * ca_create (which frees everything it allocated and leaves *out NULL) */
 const char *ca_last_error(const ca_handle *h);
 
 /* hook-level entry points (device pointers, stream-ordered) */
@@ -132,11 +134,16 @@ ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *layouts
  *                           1 and 2 need nlev + 1 <= 64.
  *   CA_TUNE_EXPLICIT_CHUNK: levels per work item of the explicit-tendency
  *                           kernel (0 = whole column).
- *   CA_TUNE_DSS_CHUNK:      levels per work item of the DSS kernel (>= 1). */
+ *   CA_TUNE_DSS_CHUNK:      levels per work item of the DSS kernel (>= 1).
+ *   CA_TUNE_DSS_KIND:       0 = staged through LDS per element cluster,
+ *                           1 = direct gather from global memory.  Both sum
+ *                           the same copies in the same order: results are
+ *                           bitwise identical. */
 typedef enum ca_tune_knob {
     CA_TUNE_NEWTON = 0,
     CA_TUNE_EXPLICIT_CHUNK = 1,
-    CA_TUNE_DSS_CHUNK = 2
+    CA_TUNE_DSS_CHUNK = 2,
+    CA_TUNE_DSS_KIND = 3
 } ca_tune_knob;
 ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value);
 ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value);

@@ -191,13 +191,15 @@ def test_sphere_one_and_five_steps():
     assert e5["u3"] <= 2.0 * fl5
 
 
+@pytest.mark.parametrize("dss_kind", [0, 1])
 @pytest.mark.parametrize("dss_chunk", [1, 2, 3, 4, 10])
-def test_small_sphere_dss_level_chunks(dss_chunk):
-    """h_elem = 2, z_elem = 10: the DSS grid (column blocks x level chunks) is
-    not a multiple of the eight XCDs for most chunk sizes; one step must match
-    the oracle for every chunking."""
+def test_small_sphere_dss_level_chunks(dss_chunk, dss_kind):
+    """h_elem = 2, z_elem = 10: the DSS grid (column blocks or element
+    clusters x level chunks) is not a multiple of the eight XCDs for most
+    chunk sizes; one step must match the oracle for every chunking, with the
+    LDS-staged (0) and the direct (1) DSS."""
     torch = _torch()
-    from cadycore.lib import CA_TUNE_DSS_CHUNK, DryDycore
+    from cadycore.lib import CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND, DryDycore
     from oracle import Oracle
 
     g = G.cubed_sphere(2, G.tanh_stretched_faces(10, 30000.0, 300.0), PAR.planet_radius, PAR.Omega)
@@ -205,6 +207,7 @@ def test_small_sphere_dss_level_chunks(dss_chunk):
     Y32 = I.sphere_state(g, PAR, seed=1).astype(np.float32).astype(np.float64)
     d = DryDycore(g, PAR, num)
     d.set_tuning(CA_TUNE_DSS_CHUNK, dss_chunk)
+    d.set_tuning(CA_TUNE_DSS_KIND, dss_kind)
     Yg = d.state(Y32)
     d.step(Yg, 100.0, 1)
     torch.cuda.synchronize()
@@ -286,6 +289,31 @@ def test_full_size_properties_h30():
             col = arr[:, members]
             assert np.all(col == col[:, :1])
     d.close()
+
+
+def test_dss_forms_bitwise_identical_h30():
+    """The LDS-staged and the direct DSS sum the same copies in the same
+    order: the dss! hook and two full steps agree bitwise at h_elem = 30."""
+    torch = _torch()
+    from cadycore.lib import CA_TUNE_DSS_KIND, DryDycore
+
+    zf = G.tanh_stretched_faces(63, 30000.0, 30.0)
+    g = G.cubed_sphere(30, zf, PAR.planet_radius, PAR.Omega)
+    num = P.Numerics(dt=60.0)
+    Y0 = I.sphere_state(g, PAR, seed=5)
+    d = DryDycore(g, PAR, num)
+    out = []
+    for kind in (0, 1):
+        d.set_tuning(CA_TUNE_DSS_KIND, kind)
+        Y = d.state(Y0)
+        d.dss(Y)
+        Z = Y.clone()
+        d.step(Z, 60.0, 2)
+        torch.cuda.synchronize()
+        out.append((Y.cpu(), Z.cpu()))
+    d.close()
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
 
 
 def test_pack_unpack_roundtrip_climacore_layout():

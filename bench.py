@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--dt", type=float, default=60.0)
     ap.add_argument("--cpu-baseline-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ragged column ensemble (512 columns, z_elem in {30, 45, 63, 90, 128}, 360 steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
@@ -115,6 +117,43 @@ def cpu_baseline(grid, params, numerics, Y0, dt, nsteps):
         "sample": f"{nsteps} full ARS343 steps of the same h{grid.h_elem} z{grid.nlev} workload (after one untimed "
                   f"step), Float32, {threads} OpenMP threads: {per_step:.3f} s/step",
     }
+
+
+def ensemble_c3(torch, params):
+    """512 independent columns of uneven vertical sizes and per-member
+    parameters, 360 ARS343 steps of 10 s in one persistent launch (after the
+    autotuner picked the schedule).  The state is cache-resident (0.7 MB), so
+    the figure is steps per second, not an HBM fraction."""
+    from cadycore.ensemble import ColumnEnsemble, c3_members
+    from cadycore.params import Numerics
+
+    num = Numerics(hyperdiff=False, dt=10.0, rayleigh=True, vert_diff=True)
+    members, states = c3_members(params, n=512, seed=3)
+    ens = ColumnEnsemble(members, params, num)
+    Y0 = ens.state(states)
+    tune = ens.autotune(Y0, num.dt, 20)
+    Y = Y0.clone()
+    ens.step(Y, num.dt, 10)
+    Y = Y0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(Y, num.dt, 360)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out = {
+        "members": len(members),
+        "levels": sorted({m.nlev for m in members}),
+        "steps": 360,
+        "ms": round(el * 1e3, 3),
+        "ensemble_steps_per_s": round(360 / el, 1),
+        "column_steps_per_s": round(360 * len(members) / el, 1),
+        "schedule": ens.get_tuning(),
+        "autotune_ms_per_20_steps": {f"lanes{k[0]}_order{k[1]}": round(v, 4) for k, v in tune.items()},
+        "finite": bool(torch.isfinite(Y).all().item()),
+        "state_MB": round(ens.S * 4 / 1e6, 3),
+    }
+    ens.close()
+    return out
 
 
 def copy_calibration(torch):
@@ -201,6 +240,7 @@ def main():
 
     if a.copy_calibration:
         copy_calibration(torch)
+    ens = None if a.no_ensemble else ensemble_c3(torch, par)
 
     cpu = None
     if not a.no_cpu_baseline and a.cpu_baseline_steps > 0:
@@ -232,6 +272,7 @@ def main():
         "finite": bool(ok),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "c3_ensemble": ens,
         "kernels": kernels,
     }
     print(json.dumps(line))

@@ -56,6 +56,11 @@ class CaConfig(C.Structure):
     ]
 
 
+class CaMember(C.Structure):
+    _fields_ = [("nlev", C.c_int32), ("z_face", C.POINTER(C.c_double)), ("alpha_w", C.c_double),
+                ("D0", C.c_double), ("f3", C.c_double)]
+
+
 class CaLayout(C.Structure):
     _fields_ = [("offset", C.c_int64), ("s_v", C.c_int64), ("s_i", C.c_int64),
                 ("s_j", C.c_int64), ("s_h", C.c_int64)]
@@ -67,8 +72,14 @@ EXPORTED = (
     "ca_correct_implicit_advection", "ca_dss", "ca_step", "ca_pack_state", "ca_unpack_state",
     "ca_check_finite", "ca_profile_enable", "ca_profile_count", "ca_profile_get",
     "ca_set_tuning", "ca_get_tuning",
+    "ca_ensemble_create", "ca_ensemble_destroy", "ca_ensemble_state_size", "ca_ensemble_member_offset",
+    "ca_ensemble_last_error", "ca_ensemble_step", "ca_ensemble_step_graph", "ca_ensemble_set_tuning",
+    "ca_ensemble_get_tuning", "ca_ensemble_autotune",
 )
 CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND = 0, 1, 2, 3
+CA_ENS_LANES, CA_ENS_ORDER = 0, 1
+_NOT_STATUS = ("ca_state_size", "ca_last_error", "ca_profile_count", "ca_ensemble_state_size",
+               "ca_ensemble_member_offset", "ca_ensemble_last_error")
 
 
 def load_library(path: str = LIB_PATH):
@@ -97,8 +108,22 @@ def load_library(path: str = LIB_PATH):
     lib.ca_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     lib.ca_profile_get.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_double)]
+    lib.ca_ensemble_create.argtypes = [C.POINTER(CaParams), C.POINTER(CaConfig), C.c_int32, C.POINTER(CaMember),
+                                       C.POINTER(vp)]
+    lib.ca_ensemble_destroy.argtypes = [vp]
+    lib.ca_ensemble_state_size.argtypes = [vp]
+    lib.ca_ensemble_state_size.restype = C.c_int64
+    lib.ca_ensemble_member_offset.argtypes = [vp, C.c_int32]
+    lib.ca_ensemble_member_offset.restype = C.c_int64
+    lib.ca_ensemble_last_error.argtypes = [vp]
+    lib.ca_ensemble_last_error.restype = C.c_char_p
+    lib.ca_ensemble_step.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
+    lib.ca_ensemble_step_graph.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
+    lib.ca_ensemble_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.ca_ensemble_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    lib.ca_ensemble_autotune.argtypes = [vp, vp, C.c_double, C.c_int32, vp, C.POINTER(C.c_double)]
     for nm in EXPORTED:
-        if nm not in ("ca_state_size", "ca_last_error", "ca_profile_count"):
+        if nm not in _NOT_STATUS:
             getattr(lib, nm).restype = C.c_int
     lib.ca_profile_count.restype = C.c_int32
     return lib
@@ -170,7 +195,8 @@ class DryDycore:
         h = C.c_void_p()
         st = self.lib.ca_create(C.byref(g), C.byref(p), C.byref(c), C.byref(h))
         if st != CA_OK:
-            raise RuntimeError(f"ca_create failed: {_STATUS.get(st, st)}")
+            msg = self.lib.ca_last_error(None)
+            raise RuntimeError(f"ca_create failed: {_STATUS.get(st, st)}: {msg.decode() if msg else ''}")
         self.h = h
         self.S = int(self.lib.ca_state_size(h))
         self._keep = keep

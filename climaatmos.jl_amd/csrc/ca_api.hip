@@ -287,6 +287,72 @@ ca_status ensure_bands(ca_handle *h) {
 }
 
 
+// Per-level tables of one column grid, computed in double and stored in
+// float (plus float64 copies of the geopotential differences): lc holds 8
+// centre rows of nl (z, Jv, s, Rayleigh zeta, Phi, 1/s^2, 1/Jv, 1/(s^2 Jv)),
+// lf 9 face rows of nl + 1 (z, Jv, s, dPhi, zeta, 1/Jv^2, 1/Jv, 1/s,
+// 1/(s^2 Jv)).  R = 0: no deep-atmosphere scaling.
+struct LevelTables {
+    std::vector<float> lc, lf;
+    std::vector<double> dphid, phicd;
+};
+LevelTables level_tables(int nl, const double *z_face, double R, double grav, double zd) {
+    LevelTables T;
+    std::vector<double> zf(z_face, z_face + nl + 1), zcd(nl);
+    T.lc.assign(8 * (size_t)nl, 0.0f);
+    T.lf.assign(9 * (size_t)(nl + 1), 0.0f);
+    const double zmax = zf[nl];
+    for (int k = 0; k < nl; ++k) zcd[k] = 0.5 * (zf[k] + zf[k + 1]);
+    auto zeta = [&](double z) {
+        if (!(z > zd)) return 0.0;
+        const double sn = std::sin(M_PI * (z - zd) / (zmax - zd) / 2);
+        return sn * sn;
+    };
+    for (int k = 0; k < nl; ++k) {
+        const double sk = R > 0 ? (R + zcd[k]) / R : 1.0, Jv = zf[k + 1] - zf[k];
+        const double v[8] = {zcd[k], Jv, sk, zeta(zcd[k]), grav * zcd[k], 1.0 / (sk * sk), 1.0 / Jv, 1.0 / (sk * sk * Jv)};
+        for (int r = 0; r < 8; ++r) T.lc[r * (size_t)nl + k] = (float)v[r];
+    }
+    const int nf = nl + 1;
+    for (int f = 0; f <= nl; ++f) {
+        const double Jvf = f == 0 ? zf[1] - zf[0] : (f == nl ? zf[nl] - zf[nl - 1] : zcd[f] - zcd[f - 1]);
+        const double sf = R > 0 ? (R + zf[f]) / R : 1.0;
+        const double v[9] = {zf[f], Jvf, sf, f >= 1 && f < nl ? grav * (zcd[f] - zcd[f - 1]) : 0.0, zeta(zf[f]),
+                             1.0 / (Jvf * Jvf), 1.0 / Jvf, 1.0 / sf, 1.0 / (sf * sf * Jvf)};
+        for (int r = 0; r < 9; ++r) T.lf[r * (size_t)nf + f] = (float)v[r];
+    }
+    T.dphid.assign(nl + 1, 0.0);
+    T.phicd.assign(nl, 0.0);
+    for (int f = 1; f < nl; ++f) T.dphid[f] = grav * (zcd[f] - zcd[f - 1]);
+    for (int k = 0; k < nl; ++k) T.phicd[k] = grav * zcd[k];
+    return T;
+}
+// points g's per-level tables at device copies laid out as in LevelTables
+void point_level_tables(DevGeom &g, int nl, const float *dlc, const float *dlf, const double *ddphi,
+                        const double *dphic) {
+    const int nf = nl + 1;
+    g.dPhif_d = ddphi;
+    g.Phic_d = dphic;
+    g.zc = dlc;
+    g.Jvc = dlc + nl;
+    g.dzc = dlc + nl;
+    g.sc = dlc + 2 * nl;
+    g.zeta_c = dlc + 3 * nl;
+    g.Phic = dlc + 4 * nl;
+    g.is2c = dlc + 5 * nl;
+    g.iJvc = dlc + 6 * nl;
+    g.isJc = dlc + 7 * nl;
+    g.zf = dlf;
+    g.Jvf = dlf + nf;
+    g.sf = dlf + 2 * nf;
+    g.dPhif = dlf + 3 * nf;
+    g.zeta_f = dlf + 4 * nf;
+    g.g33f = dlf + 5 * nf;
+    g.iJvf = dlf + 6 * nf;
+    g.isf = dlf + 7 * nf;
+    g.isJf = dlf + 8 * nf;
+}
+
 // ca_create's body: fills h; on failure the caller frees h
 ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_config *cfg, ca_handle *h) {
     const int nl = geom->nlev, nc = geom->ncol;
@@ -324,71 +390,16 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     if ((s = upload(h, &dbasis, tofloat(geom->basis, 12 * (size_t)nc)))) return s;
     g.basis = dbasis;
     // vertical tables (computed in double, stored in float)
-    std::vector<double> zf(geom->z_face, geom->z_face + nl + 1);
-    std::vector<float> lc(8 * (size_t)nl), lf(9 * (size_t)(nl + 1));
-    const double R = geom->radius;
-    const double zmax = zf[nl];
-    std::vector<double> zcd(nl);
-    for (int k = 0; k < nl; ++k) zcd[k] = 0.5 * (zf[k] + zf[k + 1]);
-    auto zeta = [&](double z) {
-        if (!(z > cfg->zd_rayleigh)) return 0.0;
-        double sn = std::sin(M_PI * (z - cfg->zd_rayleigh) / (zmax - cfg->zd_rayleigh) / 2);
-        return sn * sn;
-    };
-    for (int k = 0; k < nl; ++k) {
-        lc[0 * (size_t)nl + k] = (float)zcd[k];
-        lc[1 * (size_t)nl + k] = (float)(zf[k + 1] - zf[k]);
-        lc[2 * (size_t)nl + k] = (float)(R > 0 ? (R + zcd[k]) / R : 1.0);
-        lc[3 * (size_t)nl + k] = (float)zeta(zcd[k]);
-        lc[4 * (size_t)nl + k] = (float)(par->grav * zcd[k]);
-        const double sk = R > 0 ? (R + zcd[k]) / R : 1.0, Jv = zf[k + 1] - zf[k];
-        lc[5 * (size_t)nl + k] = (float)(1.0 / (sk * sk));
-        lc[6 * (size_t)nl + k] = (float)(1.0 / Jv);
-        lc[7 * (size_t)nl + k] = (float)(1.0 / (sk * sk * Jv));
+    {
+        const LevelTables T = level_tables(nl, geom->z_face, geom->radius, par->grav, cfg->zd_rayleigh);
+        double *ddphi = nullptr, *dphic = nullptr;
+        float *dlc = nullptr, *dlf = nullptr;
+        if ((s = upload(h, &ddphi, T.dphid))) return s;
+        if ((s = upload(h, &dphic, T.phicd))) return s;
+        if ((s = upload(h, &dlc, T.lc))) return s;
+        if ((s = upload(h, &dlf, T.lf))) return s;
+        point_level_tables(g, nl, dlc, dlf, ddphi, dphic);
     }
-    const int nf = nl + 1;
-    for (int f = 0; f <= nl; ++f) {
-        double Jvf = f == 0 ? zf[1] - zf[0] : (f == nl ? zf[nl] - zf[nl - 1] : zcd[f] - zcd[f - 1]);
-        lf[0 * (size_t)nf + f] = (float)zf[f];
-        lf[1 * (size_t)nf + f] = (float)Jvf;
-        lf[2 * (size_t)nf + f] = (float)(R > 0 ? (R + zf[f]) / R : 1.0);
-        lf[3 * (size_t)nf + f] = (float)(f >= 1 && f < nl ? par->grav * (zcd[f] - zcd[f - 1]) : 0.0);
-        lf[4 * (size_t)nf + f] = (float)zeta(zf[f]);
-        const double sf = R > 0 ? (R + zf[f]) / R : 1.0;
-        lf[5 * (size_t)nf + f] = (float)(1.0 / (Jvf * Jvf));
-        lf[6 * (size_t)nf + f] = (float)(1.0 / Jvf);
-        lf[7 * (size_t)nf + f] = (float)(1.0 / sf);
-        lf[8 * (size_t)nf + f] = (float)(1.0 / (sf * sf * Jvf));
-    }
-    std::vector<double> dphid(nl + 1, 0.0), phicd(nl);
-    for (int f = 1; f < nl; ++f) dphid[f] = par->grav * (zcd[f] - zcd[f - 1]);
-    for (int k = 0; k < nl; ++k) phicd[k] = par->grav * zcd[k];
-    double *ddphi = nullptr, *dphic = nullptr;
-    if ((s = upload(h, &ddphi, dphid))) return s;
-    if ((s = upload(h, &dphic, phicd))) return s;
-    g.dPhif_d = ddphi;
-    g.Phic_d = dphic;
-    float *dlc = nullptr, *dlf = nullptr;
-    if ((s = upload(h, &dlc, lc))) return s;
-    if ((s = upload(h, &dlf, lf))) return s;
-    g.zc = dlc;
-    g.Jvc = dlc + nl;
-    g.dzc = dlc + nl;
-    g.sc = dlc + 2 * nl;
-    g.zeta_c = dlc + 3 * nl;
-    g.Phic = dlc + 4 * nl;
-    g.is2c = dlc + 5 * nl;
-    g.iJvc = dlc + 6 * nl;
-    g.isJc = dlc + 7 * nl;
-    g.zf = dlf;
-    g.Jvf = dlf + nf;
-    g.sf = dlf + 2 * nf;
-    g.dPhif = dlf + 3 * nf;
-    g.zeta_f = dlf + 4 * nf;
-    g.g33f = dlf + 5 * nf;
-    g.iJvf = dlf + 6 * nf;
-    g.isf = dlf + 7 * nf;
-    g.isJf = dlf + 8 * nf;
     g.aw_col = nullptr;
     g.D0_col = nullptr;
     if (cfg->alpha_w_col) {

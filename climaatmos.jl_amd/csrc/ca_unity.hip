@@ -4,5 +4,7 @@
 #include "ca_newton_pcr.hip"
 #include "ca_spectral.hip"
 #include "ca_dss.hip"
+#include "ca_ensemble.hip"
 #include "ca_dss_table.cpp"
 #include "ca_api.hip"
+#include "ca_ensemble_api.hip"

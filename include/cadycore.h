@@ -162,6 +162,47 @@ int32_t ca_profile_count(void);
 ca_status ca_profile_get(ca_handle *h, int32_t id, const char **name, int64_t *launches,
                          double *total_ms);
 
+/* ------------------------------------------------------------------------
+ * Column ensembles.  No reference counterpart: the reference runs one
+ * ColumnGrid per simulation (src/simulation/grids.jl:110-129) and ensembles
+ * as separate processes (calibration/.../run_calibration.jl:89-110).  Each
+ * member is an independent column with its own levels and parameters,
+ * stepped with the same ARS343 step as ca_step on a column grid.  The
+ * ensemble state is ragged: member m's [rho | u_1 | u_2 | rho e_tot]
+ * (nlev each) and u_3 (nlev + 1) start at ca_ensemble_member_offset(e, m).
+ * The config's hyperdiffusion fields are ignored (columns have none);
+ * rayleigh and vert_diff use the members' alpha_w and D0. */
+typedef struct ca_ensemble ca_ensemble;
+typedef struct ca_member {
+    int32_t nlev;          /* centre levels, 3 .. 128                         */
+    const double *z_face;  /* [nlev + 1] face heights, increasing             */
+    double alpha_w;        /* Rayleigh coefficient on u_3 (1/s)               */
+    double D0;             /* vertical diffusivity (m^2/s)                    */
+    double f3;             /* Coriolis parameter of the column (1/s)          */
+} ca_member;
+ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, int32_t nmembers,
+                             const ca_member *members, ca_ensemble **out);
+ca_status ca_ensemble_destroy(ca_ensemble *e);
+int64_t ca_ensemble_state_size(const ca_ensemble *e);
+int64_t ca_ensemble_member_offset(const ca_ensemble *e, int32_t member);
+const char *ca_ensemble_last_error(const ca_ensemble *e);
+/* nsteps steps of every member in one persistent launch */
+ca_status ca_ensemble_step(ca_ensemble *e, float *Y, double dt, int32_t nsteps, void *stream);
+/* the same launch captured into a hipGraph (kept and replayed while Y, dt,
+ * nsteps, the schedule and the stream stay the same); stream must not be
+ * NULL (the legacy stream cannot be captured) */
+ca_status ca_ensemble_step_graph(ca_ensemble *e, float *Y, double dt, int32_t nsteps, void *stream);
+/* schedule knobs: CA_ENS_LANES = lanes per member (64, 128, 256);
+ * CA_ENS_ORDER = member-to-workgroup order (0 as given, 1 longest first) */
+typedef enum ca_ens_knob { CA_ENS_LANES = 0, CA_ENS_ORDER = 1 } ca_ens_knob;
+ca_status ca_ensemble_set_tuning(ca_ensemble *e, int32_t knob, int32_t value);
+ca_status ca_ensemble_get_tuning(const ca_ensemble *e, int32_t knob, int32_t *value);
+/* times every schedule on a device copy of Y (Y is not modified), keeps the
+ * fastest; times_ms (optional, 6 entries) receives lanes 64/128/256 x order
+ * 0/1 in that order */
+ca_status ca_ensemble_autotune(ca_ensemble *e, const float *Y, double dt, int32_t nsteps, void *stream,
+                               double *times_ms);
+
 #ifdef __cplusplus
 }
 #endif

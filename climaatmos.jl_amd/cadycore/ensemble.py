@@ -157,6 +157,39 @@ class ColumnEnsemble:
             out[name] = v.value
         return out
 
+    def set_params(self, alpha_w=None, D0=None):
+        """New per-member Rayleigh coefficients and / or diffusivities."""
+        def arr(x):
+            if x is None:
+                return None, None
+            a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+            assert a.size == len(self.members)
+            return a, a.ctypes.data_as(C.POINTER(C.c_double))
+        aw, paw = arr(alpha_w)
+        d0, pd0 = arr(D0)
+        self._check(self.lib.ca_ensemble_set_params(self.h, paw, pd0), "ca_ensemble_set_params")
+        for i, m in enumerate(self.members):
+            if aw is not None:
+                m.alpha_w = float(aw[i])
+            if d0 is not None:
+                m.D0 = float(d0[i])
+
+    def adjoint(self, Y0, YbN, dt, nsteps, every=8):
+        """Reverse mode of ``nsteps`` steps from Y0: given the cotangent YbN of
+        the states after the steps, returns the cotangents of Y0 and of each
+        member's D0 and alpha_w (device tensors)."""
+        t = self.torch
+        Yb0 = t.empty_like(Y0)
+        M = len(self.members)
+        D0b = t.empty(M, dtype=t.float32, device=self.device)
+        awb = t.empty(M, dtype=t.float32, device=self.device)
+        assert YbN.is_cuda and YbN.dtype == t.float32 and YbN.is_contiguous() and YbN.numel() == self.S
+        self._check(self.lib.ca_ensemble_adjoint(self.h, self._ptr(Y0), C.c_void_p(YbN.data_ptr()), float(dt),
+                                                 int(nsteps), int(every), C.c_void_p(Yb0.data_ptr()),
+                                                 C.c_void_p(D0b.data_ptr()), C.c_void_p(awb.data_ptr()),
+                                                 self._stream()), "ca_ensemble_adjoint")
+        return Yb0, D0b, awb
+
     def autotune(self, Y, dt, nsteps):
         """Times every schedule on a copy of Y and keeps the fastest; returns
         {(lanes, order): ms}."""

@@ -74,7 +74,8 @@ EXPORTED = (
     "ca_set_tuning", "ca_get_tuning",
     "ca_ensemble_create", "ca_ensemble_destroy", "ca_ensemble_state_size", "ca_ensemble_member_offset",
     "ca_ensemble_last_error", "ca_ensemble_step", "ca_ensemble_step_graph", "ca_ensemble_set_tuning",
-    "ca_ensemble_get_tuning", "ca_ensemble_autotune",
+    "ca_ensemble_get_tuning", "ca_ensemble_autotune", "ca_ensemble_set_params", "ca_ensemble_adjoint",
+    "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint",
 )
 CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND = 0, 1, 2, 3
 CA_ENS_LANES, CA_ENS_ORDER = 0, 1
@@ -122,6 +123,12 @@ def load_library(path: str = LIB_PATH):
     lib.ca_ensemble_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
     lib.ca_ensemble_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     lib.ca_ensemble_autotune.argtypes = [vp, vp, C.c_double, C.c_int32, vp, C.POINTER(C.c_double)]
+    dpp = C.POINTER(C.c_double)
+    lib.ca_ensemble_set_params.argtypes = [vp, dpp, dpp]
+    lib.ca_ensemble_adjoint.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]
+    lib.ca_dss_vjp.argtypes = [vp, vp, vp]
+    lib.ca_remaining_tendency_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.ca_step_adjoint.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, vp]
     for nm in EXPORTED:
         if nm not in _NOT_STATUS:
             getattr(lib, nm).restype = C.c_int
@@ -261,6 +268,28 @@ class DryDycore:
 
     def step(self, Y, dt, nsteps=1):
         self._check(self.lib.ca_step(self.h, self._ptr(Y), float(dt), int(nsteps), self._stream()), "step!")
+
+    # ------------------------------------------------------- reverse mode
+    def dss_vjp(self, Yb):
+        """Yb <- DSS^T Yb (in place): the transpose of dss!."""
+        self._check(self.lib.ca_dss_vjp(self.h, self._ptr(Yb), self._stream()), "dss! reverse")
+
+    def remaining_tendency_vjp(self, Yb, Ytb, Y, D0b=None):
+        """Yb <- (d T_exp / d Y)^T Ytb at Y, and (vertical diffusion) D0b <-
+        d <Ytb, T_exp> / d D0 per column; sphere grids."""
+        d0 = C.c_void_p(D0b.data_ptr()) if D0b is not None else None
+        self._check(self.lib.ca_remaining_tendency_vjp(self.h, self._ptr(Yb), self._ptr(Ytb), self._ptr(Y), d0,
+                                                       self._stream()), "remaining_tendency! reverse")
+
+    def step_adjoint(self, Y0, YbN, dt, nsteps, every=8, D0b=None):
+        """Reverse mode of ``step(Y0, dt, nsteps)``: given the cotangent YbN of
+        the state after the steps, returns the cotangent of Y0 (and fills
+        D0b, [ncol] float32, with the D0 cotangent); sphere grids."""
+        Yb0 = self.zeros()
+        d0 = C.c_void_p(D0b.data_ptr()) if D0b is not None else None
+        self._check(self.lib.ca_step_adjoint(self.h, self._ptr(Y0), self._ptr(YbN), float(dt), int(nsteps),
+                                             int(every), self._ptr(Yb0), d0, self._stream()), "step adjoint")
+        return Yb0
 
     def check_finite(self, Y):
         flag = C.c_int32(0)

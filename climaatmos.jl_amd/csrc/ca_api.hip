@@ -64,6 +64,17 @@ struct ca_handle {
     int explicit_chunk = 16;
     int dss_chunk = 8;
     int dss_kind = 0;  // 0: LDS-staged per element cluster, 1: direct gather
+    // reverse mode (ca_step_adjoint and the *_vjp hooks)
+    DevGeom gT{};                 // g with the transposed DSS tables
+    DevGeom *cgeo = nullptr;      // per-column one-column views of g (Newton stage reverse)
+    int adj_chunk = 8;            // levels per work item of k_explicit_vjp
+    float *aw_buf = nullptr;      // 13 S-sized cotangent / scratch planes
+    float *aw_L = nullptr;        // 4 Laplacian-shaped planes: Lb, Lb2, fa | fb
+    double *aw_d0 = nullptr;      // D0 cotangent partials [chunks x ncol]
+    float *aw_rec = nullptr;      // per-step records of one checkpoint segment
+    int64_t aw_rec_steps = 0;
+    float *aw_ck = nullptr;       // checkpoints
+    int64_t aw_ck_n = 0;
 };
 
 namespace {
@@ -427,6 +438,18 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
         g.gtw = reinterpret_cast<const float4 *>(d_tw);
         g.gtM = reinterpret_cast<const float4 *>(d_tM);
         g.ncl = (int)T.nE.size();
+        // the transposed (adjoint) DSS: same slots and clusters, own weights, transposed transforms
+        DssTable TT;
+        const std::string msgT = ca_build_dss_table(geom->nelem, nc, geom->nelem > 0 ? geom->n_dss : 0, geom->dss_ptr,
+                                                    geom->dss_col, geom->dss_w, geom->basis, TT, true);
+        if (!msgT.empty()) return fail(h, CA_ERR_ARG, "ca_create: " + msgT);
+        float *dT_tw = nullptr, *dT_tM = nullptr, *dT_cw = nullptr, *dT_cM = nullptr;
+        if ((s = upload(h, &dT_tw, TT.tw))) return s;
+        if ((s = upload(h, &dT_tM, TT.tM))) return s;
+        if (g.ncl > 0) {
+            if ((s = upload(h, &dT_cw, TT.cw))) return s;
+            if ((s = upload(h, &dT_cM, TT.cM))) return s;
+        }
         if (g.ncl > 0) {
             int *d_list = nullptr, *d_nE = nullptr, *d_ccol = nullptr;
             uint16_t *d_slot = nullptr;
@@ -443,6 +466,13 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
             g.dl_slot = reinterpret_cast<const ushort4 *>(d_slot);
             g.dl_w = reinterpret_cast<const float4 *>(d_cw);
             g.dl_M = reinterpret_cast<const float4 *>(d_cM);
+        }
+        h->gT = g;
+        h->gT.gtw = reinterpret_cast<const float4 *>(dT_tw);
+        h->gT.gtM = reinterpret_cast<const float4 *>(dT_tM);
+        if (g.ncl > 0) {
+            h->gT.dl_w = reinterpret_cast<const float4 *>(dT_cw);
+            h->gT.dl_M = reinterpret_cast<const float4 *>(dT_cM);
         }
     }
     float W[16], iW[16], Dm[4][4];
@@ -503,6 +533,89 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     HIPCHK(h, hipMemset(h->L, 0, 4 * (size_t)g.nC * sizeof(float)));
     ars343(h);
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
+    return CA_OK;
+}
+
+// Records of one step for the reverse pass: U_0 (the step's start state),
+// U~_i and U_i of the three implicit stages, and the DSSed Laplacians L_i at
+// U_0 .. U_3 (hyperdiffusion only).
+inline int64_t rec_size(const ca_handle *h) { return 7 * h->S + 16 * h->g.nC; }
+inline float *rec_U(const ca_handle *h, float *rec, int i) { return rec + (i == 0 ? 0 : 3 + i) * h->S; }
+inline float *rec_Ut(const ca_handle *h, float *rec, int i) { return rec + (int64_t)i * h->S; }
+inline float *rec_L(const ca_handle *h, float *rec, int i) { return rec + 7 * h->S + (int64_t)i * 4 * h->g.nC; }
+
+// One ARS343 step of Y in place; Y is continuous and filtered on entry and on
+// exit.  With rec, the step's records are written there as well.
+ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec) {
+    const DevGeom &g = h->g;
+    ca_status s;
+    // stage 1: explicit tendency at u (DSSed)
+    if (rec) HIPCHK(h, hipMemcpyAsync(rec_U(h, rec, 0), Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if ((s = explicit_tendency(h, h->FE[0], Y, st, true))) return s;
+    if (rec && h->n.hyper)
+        HIPCHK(h, hipMemcpyAsync(rec_L(h, rec, 0), h->L, 4 * (size_t)g.nC * sizeof(float), hipMemcpyDeviceToDevice, st));
+    for (int i = 1; i < 4; ++i) {
+        StageSrc src{};
+        src.full[0] = Y;
+        src.cf[0] = 1.0f;
+        src.nfull = 1;
+        for (int j = 0; j < i; ++j)
+            if (h->aE[i][j] != 0.0) {
+                src.full[src.nfull] = h->FE[j];
+                src.cf[src.nfull] = (float)(dt * h->aE[i][j]);
+                ++src.nfull;
+            }
+        src.nfi = 0;
+        for (int j = 1; j < i; ++j)
+            if (h->aI[i][j] != 0.0) {
+                src.fi[src.nfi] = h->FI[j];
+                src.ci[src.nfi] = (float)(dt * h->aI[i][j]);
+                ++src.nfi;
+            }
+        float dtg = (float)(dt * h->aI[i][i]);
+        if (rec) {
+            sadj::k_stage_state<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, src, rec_Ut(h, rec, i));
+            if ((s = check_launch(h, "k_stage_state"))) return s;
+        }
+        {
+            Launch l(h, st, KID_NEWTON_STAGE);
+            if (h->newton_variant == 1 || h->newton_variant == 2)
+                launch_newton_pcr(h, src, dtg, i, st);
+            else
+                k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i], h->scr);
+        }
+        if ((s = check_launch(h, "k_newton_stage"))) return s;
+        if (rec)
+            HIPCHK(h, hipMemcpyAsync(rec_U(h, rec, i), h->U, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
+        if ((s = explicit_tendency(h, h->FE[i], h->U, st, true))) return s;
+        if (rec && h->n.hyper)
+            HIPCHK(h, hipMemcpyAsync(rec_L(h, rec, i), h->L, 4 * (size_t)g.nC * sizeof(float), hipMemcpyDeviceToDevice,
+                                     st));
+    }
+    // u_{n+1} = u + dt sum b (F_E + F_I), then dss! and the filter
+    FinalSrc fs{};
+    fs.dt = (float)dt;
+    for (int j = 0; j < 4; ++j)
+        if (h->b[j] != 0.0) {
+            fs.fe[fs.nfe] = h->FE[j];
+            fs.be[fs.nfe] = (float)h->b[j];
+            ++fs.nfe;
+            if (j >= 1) {
+                fs.fi[fs.nfi] = h->FI[j];
+                fs.bi[fs.nfi] = (float)h->b[j];
+                ++fs.nfi;
+            }
+        }
+    {
+        Launch l(h, st, KID_FINAL);
+        k_combine<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(g, Y, h->U, fs);
+    }
+    if ((s = check_launch(h, "k_combine"))) return s;
+    if (g.nelem > 0) {
+        if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
+    } else {
+        if ((s = filter_columns(h, h->U, Y, st))) return s;
+    }
     return CA_OK;
 }
 
@@ -631,63 +744,8 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
         if ((s = check_launch(h, "k_combine"))) return s;
         if ((s = filter_columns(h, h->U, Y, st))) return s;
     }
-    for (int stp = 0; stp < nsteps; ++stp) {
-        // stage 1: explicit tendency at u (DSSed)
-        if ((s = explicit_tendency(h, h->FE[0], Y, st, true))) return s;
-        for (int i = 1; i < 4; ++i) {
-            StageSrc src{};
-            src.full[0] = Y;
-            src.cf[0] = 1.0f;
-            src.nfull = 1;
-            for (int j = 0; j < i; ++j)
-                if (h->aE[i][j] != 0.0) {
-                    src.full[src.nfull] = h->FE[j];
-                    src.cf[src.nfull] = (float)(dt * h->aE[i][j]);
-                    ++src.nfull;
-                }
-            src.nfi = 0;
-            for (int j = 1; j < i; ++j)
-                if (h->aI[i][j] != 0.0) {
-                    src.fi[src.nfi] = h->FI[j];
-                    src.ci[src.nfi] = (float)(dt * h->aI[i][j]);
-                    ++src.nfi;
-                }
-            float dtg = (float)(dt * h->aI[i][i]);
-            {
-                Launch l(h, st, KID_NEWTON_STAGE);
-                if (h->newton_variant == 1 || h->newton_variant == 2)
-                    launch_newton_pcr(h, src, dtg, i, st);
-                else
-                    k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i], h->scr);
-            }
-            if ((s = check_launch(h, "k_newton_stage"))) return s;
-            if ((s = explicit_tendency(h, h->FE[i], h->U, st, true))) return s;
-        }
-        // u_{n+1} = u + dt sum b (F_E + F_I), then dss! and the filter
-        FinalSrc fs{};
-        fs.dt = (float)dt;
-        for (int j = 0; j < 4; ++j)
-            if (h->b[j] != 0.0) {
-                fs.fe[fs.nfe] = h->FE[j];
-                fs.be[fs.nfe] = (float)h->b[j];
-                ++fs.nfe;
-                if (j >= 1) {
-                    fs.fi[fs.nfi] = h->FI[j];
-                    fs.bi[fs.nfi] = (float)h->b[j];
-                    ++fs.nfi;
-                }
-            }
-        {
-            Launch l(h, st, KID_FINAL);
-            k_combine<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(g, Y, h->U, fs);
-        }
-        if ((s = check_launch(h, "k_combine"))) return s;
-        if (g.nelem > 0) {
-            if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
-        } else {
-            if ((s = filter_columns(h, h->U, Y, st))) return s;
-        }
-    }
+    for (int stp = 0; stp < nsteps; ++stp)
+        if ((s = one_step(h, Y, dt, st, nullptr))) return s;
     return CA_OK;
 }
 

@@ -97,7 +97,7 @@ std::string build_clusters(int ne, DssTable &T) {
 }  // namespace
 
 std::string ca_build_dss_table(int nelem, int ncol, int ngroups, const int32_t *dss_ptr, const int32_t *dss_col,
-                               const double *dss_w, const double *basis, DssTable &T) {
+                               const double *dss_w, const double *basis, DssTable &T, bool transpose) {
     const int nc = ncol;
     std::vector<int> cgrp(nc, -1);
     for (int gi = 0; gi < ngroups; ++gi) {
@@ -123,17 +123,29 @@ std::string ca_build_dss_table(int nelem, int ncol, int ngroups, const int32_t *
             continue;
         }
         const int b0 = dss_ptr[gi], m = dss_ptr[gi + 1] - b0;
+        double wown = 0.0;
+        for (int t = 0; t < m; ++t)
+            if (dss_col[b0 + t] == c) wown = dss_w[b0 + t];
         for (int t = 0; t < m; ++t) {
             const int mcol = dss_col[b0 + t];
-            const double w = dss_w[b0 + t];
+            // forward: y_c = sum_t w_t M(c <- c_t) x_{c_t}, M_ij = a_i(c) . a^j(c_t).
+            // transpose: xbar_c = sum_t w_c M(c_t <- c)^T ybar_{c_t}, entry (j, i) = a_i(c_t) . a^j(c)
+            const double w = transpose ? wown : dss_w[b0 + t];
             T.tc[4 * (size_t)c + t] = mcol;
             T.tw[4 * (size_t)c + t] = (float)w;
             double m00 = 0, m01 = 0, m10 = 0, m11 = 0;
             for (int d = 0; d < 3; ++d) {
-                m00 += bas(6 + d, c) * bas(d, mcol);
-                m01 += bas(6 + d, c) * bas(3 + d, mcol);
-                m10 += bas(9 + d, c) * bas(d, mcol);
-                m11 += bas(9 + d, c) * bas(3 + d, mcol);
+                if (transpose) {
+                    m00 += bas(6 + d, mcol) * bas(d, c);
+                    m01 += bas(9 + d, mcol) * bas(d, c);
+                    m10 += bas(6 + d, mcol) * bas(3 + d, c);
+                    m11 += bas(9 + d, mcol) * bas(3 + d, c);
+                } else {
+                    m00 += bas(6 + d, c) * bas(d, mcol);
+                    m01 += bas(6 + d, c) * bas(3 + d, mcol);
+                    m10 += bas(9 + d, c) * bas(d, mcol);
+                    m11 += bas(9 + d, c) * bas(3 + d, mcol);
+                }
             }
             float *Mt = &T.tM[16 * (size_t)c + 4 * t];
             Mt[0] = (float)(w * m00);

@@ -28,6 +28,10 @@ struct DssTable {
 // Returns "" on success, otherwise what is wrong with the input.
 // dss_ptr / dss_col / dss_w: shared-node groups (CSR, member columns,
 // weights WJ / sum WJ); basis: 12 planes of ncol (a^1, a^2, a_1, a_2 as
-// Cartesian x, y, z); nelem = 0 for independent columns.
+// Cartesian x, y, z); nelem = 0 for independent columns.  transpose: the
+// tables of the transposed (adjoint) DSS instead -- the same slots, every
+// slot weighted by the own column's weight, each transform the transpose of
+// the one the slot's column applies to the own column's copy.  The
+// clusters and slot positions are the same as the forward table's.
 std::string ca_build_dss_table(int nelem, int ncol, int ngroups, const int32_t *dss_ptr, const int32_t *dss_col,
-                               const double *dss_w, const double *basis, DssTable &out);
+                               const double *dss_w, const double *basis, DssTable &out, bool transpose = false);

@@ -30,25 +30,27 @@ namespace ens {
 
 // LDS of one member.  The scratch union holds the explicit-tendency or the
 // Newton-stage intermediates.
-struct Shared {
-    float u[5][CA_ENS_NF];      // state at the start of the step (4 centre planes, u_3)
-    float fe[4][5][CA_ENS_NF];  // explicit tendencies of the four stages
-    float fi[3][3][CA_ENS_NF];  // implicit tendencies of stages 1..3: rho, rho e_tot, u_3
-    float U[5][CA_ENS_NF];      // stage state
+template <int NF>
+struct SharedT {
+    float u[5][NF];      // state at the start of the step (4 centre planes, u_3)
+    float fe[4][5][NF];  // explicit tendencies of the four stages
+    float fi[3][3][NF];  // implicit tendencies of stages 1..3: rho, rho e_tot, u_3
+    float U[5][NF];      // stage state
     union {
         struct {
-            float c[10][CA_ENS_NF];  // rhoJ, Kh, Kv, u1, u2, rho, sd, irhoJ, d1, d2
-            float f[5][CA_ENS_NF];   // G1, G2, Fs, Fu0, Fu1
+            float c[10][NF];  // rhoJ, Kh, Kv, u1, u2, rho, sd, irhoJ, d1, d2
+            float f[5][NF];   // G1, G2, Fs, Fu0, Fu1
         } ex;
         struct {
-            float c[11][CA_ENS_NF];    // rho, K, T, p, Pi, ct1, ct2, h, J, iJ, rJ
-            double cd[3][CA_ENS_NF];   // Td, pd, Pid
-            float row[6][CA_ENS_NF];   // a0, a1, e0, e1, Rr, Re
-            double sys[2][4][CA_ENS_NF];  // PCR ping-pong: L, D, U, r
-            float x[CA_ENS_NF];        // du_3
-            float dF[CA_ENS_NF];       // upwind - central flux difference
-            float hr[CA_ENS_NF];       // h relative to the lowest cell, new state
-            float rJ[CA_ENS_NF];       // rho J, new state
+            float c[11][NF];    // rho, K, T, p, Pi, ct1, ct2, h, J, iJ, rJ
+            double cd[3][NF];   // Td, pd, Pid
+            float row[6][NF];   // a0, a1, e0, e1, Rr, Re
+            double sys[2][4][NF];  // PCR ping-pong: L, D, U, r
+            float x[NF];        // du_3
+            float dF[NF];       // upwind - central flux difference
+            float hr[NF];       // h relative to the lowest cell, new state
+            float rJ[NF];       // rho J, new state
+            float ve[NF];       // rho e_tot after the Newton update, before the correction
         } nw;
     };
 };
@@ -59,7 +61,8 @@ enum { NC_RHO, NC_K, NC_T, NC_P, NC_PI, NC_CT1, NC_CT2, NC_H, NC_J, NC_IJ, NC_RJ
 __device__ __forceinline__ void sync() { __syncthreads(); }
 
 // CV of centre k from the Newton scratch
-__device__ __forceinline__ CV cv_from(const Shared &S, int k) {
+template <int NF>
+__device__ __forceinline__ CV cv_from(const SharedT<NF> &S, int k) {
     CV c;
     c.rho = S.nw.c[NC_RHO][k];
     c.K = S.nw.c[NC_K][k];
@@ -82,9 +85,9 @@ __device__ __forceinline__ CV cv_from(const Shared &S, int k) {
 // branch of advection.jl:262-283, Rayleigh u_h sponge, K-theory vertical
 // diffusion): Y (LDS planes) -> T (LDS planes).  Same arithmetic as the
 // column path of k_explicit.
-template <int LPM>
-__device__ void explicit_tendency(Shared &S, const DevGeom &g, const DevNum &n, const float (*Y)[CA_ENS_NF],
-                                  float (*T)[CA_ENS_NF], int tid) {
+template <int LPM, int NF>
+__device__ void explicit_tendency(SharedT<NF> &S, const DevGeom &g, const DevNum &n, const float (*Y)[NF],
+                                  float (*T)[NF], int tid) {
     const int nl = g.nlev;
     const float f3n = g.f3n[0], f12a = g.f12a[0], f12b = g.f12b[0];
     float Kf = 0.0f;
@@ -186,8 +189,8 @@ __device__ void explicit_tendency(Shared &S, const DevGeom &g, const DevNum &n, 
 // F_I = (U - U~) / dtg for rho, rho e_tot and u_3.  Same algorithm as
 // k_newton_pcr (one Newton iteration from U~, manual Jacobian, Schur
 // complement on u_3, post-Newton upwind correction).
-template <int LPM>
-__device__ void newton_stage(Shared &S, const DevGeom &g, const DevNum &n, float dtg, float (*fi)[CA_ENS_NF],
+template <int LPM, int NF>
+__device__ void newton_stage(SharedT<NF> &S, const DevGeom &g, const DevNum &n, float dtg, float (*fi)[NF],
                              int tid) {
     const int nl = g.nlev;
     const float idtg = 1.0f / dtg;
@@ -312,6 +315,7 @@ __device__ void newton_stage(Shared &S, const DevGeom &g, const DevNum &n, float
         const float de = S.nw.row[2][k] * x0 + S.nw.row[3][k] * x1 - S.nw.row[5][k];
         S.U[0][k] -= dr;
         S.U[3][k] -= de;
+        S.nw.ve[k] = S.U[3][k];
         fi[0][k] = -dr * idtg;
         fi[1][k] = -de * idtg;
     }
@@ -351,15 +355,67 @@ __device__ void newton_stage(Shared &S, const DevGeom &g, const DevNum &n, float
     }
 }
 
+// One ARS343 step of the member in S.u (in place).  With rUt / rUs the
+// stage states U~_i and U_i are also written there (the adjoint's recompute).
+template <int LPM, int NF>
+__device__ void step_once(SharedT<NF> &S, const DevGeom &g, const DevNum &n, const EnsTab &tab, float dt, int tid,
+                          float (*rUt)[5][NF] = nullptr, float (*rUs)[5][NF] = nullptr) {
+    const int nl = g.nlev;
+    explicit_tendency<LPM>(S, g, n, S.u, S.fe[0], tid);
+    for (int i = 1; i < 4; ++i) {
+        // stage state U~_i = u + dt sum_j (aE F_E_j + aI F_I_j)
+        for (int q = tid; q < 5 * NF; q += LPM) {
+            const int p = q / NF, k = q - p * NF;
+            if (k > (p < 4 ? nl - 1 : nl)) continue;
+            float s = 0.0f;
+            for (int j = 0; j < i; ++j) {
+                s = fmaf(tab.aE[i][j], S.fe[j][p][k], s);
+                if (j >= 1 && (p == 0 || p == 3 || p == 4))
+                    s = fmaf(tab.aI[i][j], S.fi[j - 1][p == 0 ? 0 : (p == 3 ? 1 : 2)][k], s);
+            }
+            float v = fmaf(dt, s, S.u[p][k]);
+            if (p == 4 && (k == 0 || k == nl)) v = 0.0f;
+            S.U[p][k] = v;
+            if (rUt) rUt[i - 1][p][k] = v;
+        }
+        sync();
+        newton_stage<LPM>(S, g, n, dt * tab.aI[i][i], S.fi[i - 1], tid);
+        if (rUs) {
+            for (int q = tid; q < 5 * NF; q += LPM) {
+                const int p = q / NF, k = q - p * NF;
+                if (k > (p < 4 ? nl - 1 : nl)) continue;
+                rUs[i - 1][p][k] = S.U[p][k];
+            }
+        }
+        explicit_tendency<LPM>(S, g, n, S.U, S.fe[i], tid);
+    }
+    // u <- u + dt sum_j b_j (F_E_j + F_I_j), filter u_3
+    for (int q = tid; q < 5 * NF; q += LPM) {
+        const int p = q / NF, k = q - p * NF;
+        if (k > (p < 4 ? nl - 1 : nl)) continue;
+        float s = 0.0f;
+        for (int j = 0; j < 4; ++j) {
+            if (tab.b[j] == 0.0f) continue;
+            float v = S.fe[j][p][k];
+            if (j >= 1 && (p == 0 || p == 3 || p == 4)) v += S.fi[j - 1][p == 0 ? 0 : (p == 3 ? 1 : 2)][k];
+            s = fmaf(tab.b[j], v, s);
+        }
+        float v = fmaf(dt, s, S.u[p][k]);
+        if (p == 4 && (k == 0 || k == nl)) v = 0.0f;
+        S.u[p][k] = v;
+    }
+    sync();
+}
+
 }  // namespace ens
 
 // Persistent ensemble step: workgroup b steps member order[b] nsteps times.
-template <int LPM>
+template <int LPM, int NF>
 __global__ void __launch_bounds__(LPM) k_ensemble(const DevGeom *__restrict__ mg, DevNum n, const int *__restrict__ order,
                                                   const int64_t *__restrict__ off, float *__restrict__ Y, EnsTab tab,
                                                   float dt, int nsteps) {
     using namespace ens;
-    __shared__ Shared S;
+    __shared__ SharedT<NF> S;
     const int m = order[blockIdx.x];
     const DevGeom &g = mg[m];
     const int nl = g.nlev;
@@ -370,52 +426,21 @@ __global__ void __launch_bounds__(LPM) k_ensemble(const DevGeom *__restrict__ mg
         for (int p = 0; p < 4; ++p) S.u[p][k] = Ym[p * nl + k];
     for (int f = tid; f <= nl; f += LPM) S.u[4][f] = (f == 0 || f == nl) ? 0.0f : Ym[4 * nl + f];
     sync();
-    for (int st = 0; st < nsteps; ++st) {
-        explicit_tendency<LPM>(S, g, n, S.u, S.fe[0], tid);
-        for (int i = 1; i < 4; ++i) {
-            // stage state U~_i = u + dt sum_j (aE F_E_j + aI F_I_j)
-            for (int q = tid; q < 5 * CA_ENS_NF; q += LPM) {
-                const int p = q / CA_ENS_NF, k = q - p * CA_ENS_NF;
-                if (k > (p < 4 ? nl - 1 : nl)) continue;
-                float s = 0.0f;
-                for (int j = 0; j < i; ++j) {
-                    s = fmaf(tab.aE[i][j], S.fe[j][p][k], s);
-                    if (j >= 1 && (p == 0 || p == 3 || p == 4))
-                        s = fmaf(tab.aI[i][j], S.fi[j - 1][p == 0 ? 0 : (p == 3 ? 1 : 2)][k], s);
-                }
-                float v = fmaf(dt, s, S.u[p][k]);
-                if (p == 4 && (k == 0 || k == nl)) v = 0.0f;
-                S.U[p][k] = v;
-            }
-            sync();
-            newton_stage<LPM>(S, g, n, dt * tab.aI[i][i], S.fi[i - 1], tid);
-            explicit_tendency<LPM>(S, g, n, S.U, S.fe[i], tid);
-        }
-        // u <- u + dt sum_j b_j (F_E_j + F_I_j), filter u_3
-        for (int q = tid; q < 5 * CA_ENS_NF; q += LPM) {
-            const int p = q / CA_ENS_NF, k = q - p * CA_ENS_NF;
-            if (k > (p < 4 ? nl - 1 : nl)) continue;
-            float s = 0.0f;
-            for (int j = 0; j < 4; ++j) {
-                if (tab.b[j] == 0.0f) continue;
-                float v = S.fe[j][p][k];
-                if (j >= 1 && (p == 0 || p == 3 || p == 4)) v += S.fi[j - 1][p == 0 ? 0 : (p == 3 ? 1 : 2)][k];
-                s = fmaf(tab.b[j], v, s);
-            }
-            float v = fmaf(dt, s, S.u[p][k]);
-            if (p == 4 && (k == 0 || k == nl)) v = 0.0f;
-            S.u[p][k] = v;
-        }
-        sync();
-    }
+    for (int st = 0; st < nsteps; ++st) step_once<LPM>(S, g, n, tab, dt, tid);
     for (int k = tid; k < nl; k += LPM)
         for (int p = 0; p < 4; ++p) Ym[p * nl + k] = S.u[p][k];
     for (int f = tid; f <= nl; f += LPM) Ym[4 * nl + f] = S.u[4][f];
 }
 
-template __global__ void k_ensemble<64>(const DevGeom *, DevNum, const int *, const int64_t *, float *, EnsTab, float,
-                                        int);
-template __global__ void k_ensemble<128>(const DevGeom *, DevNum, const int *, const int64_t *, float *, EnsTab,
-                                         float, int);
-template __global__ void k_ensemble<256>(const DevGeom *, DevNum, const int *, const int64_t *, float *, EnsTab,
-                                         float, int);
+#define CA_ENS_INST(LPM, NF)                                                                                    \
+    template __global__ void k_ensemble<LPM, NF>(const DevGeom *, DevNum, const int *, const int64_t *, float *, EnsTab, \
+                                                 float, int);
+CA_ENS_INST(64, 33)
+CA_ENS_INST(128, 33)
+CA_ENS_INST(64, 65)
+CA_ENS_INST(128, 65)
+CA_ENS_INST(256, 65)
+CA_ENS_INST(64, 129)
+CA_ENS_INST(128, 129)
+CA_ENS_INST(256, 129)
+#undef CA_ENS_INST

@@ -17,6 +17,7 @@ struct ca_ensemble {
     int64_t *doff = nullptr;
     int lanes = 64;
     int order_kind = 1;
+    int nf = CA_ENS_NF;  // LDS face rows per member: 33, 65 or 129, from the tallest member
     std::string err;
     // one captured launch, replayed while (Y, dt, nsteps, schedule, stream) match
     hipGraph_t graph = nullptr;
@@ -25,6 +26,13 @@ struct ca_ensemble {
     double gdt = 0.0;
     int gn = 0, glanes = 0, gorder = -1;
     hipStream_t gst = nullptr;
+    // per-member column constants (13 floats each) and their host copy
+    float *cols = nullptr;
+    std::vector<float> hcols;
+    // adjoint scratch: checkpoints and segment states
+    float *ckpt = nullptr, *segb = nullptr;
+    int64_t *ckoff = nullptr, *segoff = nullptr;
+    int ck_nsteps = -1, ck_every = -1;
 };
 
 namespace {
@@ -61,12 +69,20 @@ void ens_drop_graph(ca_ensemble *e) {
 ca_status ens_launch(ca_ensemble *e, float *Y, float dt, int nsteps, hipStream_t st) {
     const int *ord = e->order[e->order_kind];
     const dim3 grid(e->M);
-    if (e->lanes == 64)
-        k_ensemble<64><<<grid, 64, 0, st>>>(e->geoms, e->n, ord, e->doff, Y, e->tab, dt, nsteps);
-    else if (e->lanes == 128)
-        k_ensemble<128><<<grid, 128, 0, st>>>(e->geoms, e->n, ord, e->doff, Y, e->tab, dt, nsteps);
-    else
-        k_ensemble<256><<<grid, 256, 0, st>>>(e->geoms, e->n, ord, e->doff, Y, e->tab, dt, nsteps);
+#define CA_ENS_GO(LPM, NF) k_ensemble<LPM, NF><<<grid, LPM, 0, st>>>(e->geoms, e->n, ord, e->doff, Y, e->tab, dt, nsteps)
+    if (e->nf == 33) {  // at most 32 levels: 256 lanes would idle, run 128
+        if (e->lanes == 64) CA_ENS_GO(64, 33);
+        else CA_ENS_GO(128, 33);
+    } else if (e->nf == 65) {
+        if (e->lanes == 64) CA_ENS_GO(64, 65);
+        else if (e->lanes == 128) CA_ENS_GO(128, 65);
+        else CA_ENS_GO(256, 65);
+    } else {
+        if (e->lanes == 64) CA_ENS_GO(64, 129);
+        else if (e->lanes == 128) CA_ENS_GO(128, 129);
+        else CA_ENS_GO(256, 129);
+    }
+#undef CA_ENS_GO
     EHIPCHK(e, hipGetLastError());
     return CA_OK;
 }
@@ -149,6 +165,8 @@ ca_status ens_create_impl(ca_ensemble *e, const ca_params *par, const ca_config 
         S += 5 * (int64_t)nl + 1;
     }
     e->S = S;
+    const int maxl = *std::max_element(e->nlev.begin(), e->nlev.end());
+    e->nf = maxl <= 32 ? 33 : (maxl <= 64 ? 65 : CA_ENS_NF);
     float *dlc = nullptr, *dlf = nullptr, *dcol = nullptr;
     double *ddp = nullptr, *dpc = nullptr;
     ca_status s;
@@ -157,6 +175,8 @@ ca_status ens_create_impl(ca_ensemble *e, const ca_params *par, const ca_config 
     if ((s = eupload(e, &ddp, dphid))) return s;
     if ((s = eupload(e, &dpc, phicd))) return s;
     if ((s = eupload(e, &dcol, col))) return s;
+    e->cols = dcol;
+    e->hcols = col;
     std::vector<DevGeom> geoms(M);
     for (int m = 0; m < M; ++m) {
         DevGeom &g = geoms[m];
@@ -216,6 +236,8 @@ ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, i
 ca_status ca_ensemble_destroy(ca_ensemble *e) {
     if (!e) return CA_ERR_ARG;
     ens_drop_graph(e);
+    for (void *p : {(void *)e->ckpt, (void *)e->segb, (void *)e->ckoff, (void *)e->segoff})
+        if (p) (void)hipFree(p);
     for (void *p : e->allocs) (void)hipFree(p);
     delete e;
     return CA_OK;
@@ -332,6 +354,68 @@ ca_status ca_ensemble_autotune(ca_ensemble *e, const float *Y, double dt, int32_
     if (b) (void)hipEventDestroy(b);
     (void)hipFree(work);
     return s;
+}
+
+ca_status ca_ensemble_set_params(ca_ensemble *e, const double *alpha_w, const double *D0) {
+    if (!e) return CA_ERR_ARG;
+    for (int m = 0; m < e->M; ++m) {
+        if (alpha_w) e->hcols[13 * (size_t)m + 11] = (float)alpha_w[m];
+        if (D0) e->hcols[13 * (size_t)m + 12] = (float)D0[m];
+    }
+    EHIPCHK(e, hipMemcpy(e->cols, e->hcols.data(), e->hcols.size() * sizeof(float), hipMemcpyHostToDevice));
+    ens_drop_graph(e);
+    return CA_OK;
+}
+
+ca_status ca_ensemble_adjoint(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int32_t nsteps,
+                              int32_t every, float *Yb0, float *D0b, float *awb, void *stream) {
+    if (!e || !Y0 || !YbN || !Yb0 || !D0b || !awb || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    if (e->ck_nsteps != nsteps || e->ck_every != every) {
+        for (float **p : {&e->ckpt, &e->segb})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        for (int64_t **p : {&e->ckoff, &e->segoff})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        const int nseg = (nsteps + every - 1) / every;
+        std::vector<int64_t> ck(e->M), sg(e->M);
+        int64_t c = 0, g = 0;
+        for (int m = 0; m < e->M; ++m) {
+            const int64_t Sm = 5 * (int64_t)e->nlev[m] + 1;
+            ck[m] = c;
+            sg[m] = g;
+            c += (nseg + 1) * Sm;
+            g += every * Sm;
+        }
+        EHIPCHK(e, hipMalloc(&e->ckpt, c * sizeof(float)));
+        EHIPCHK(e, hipMalloc(&e->segb, g * sizeof(float)));
+        EHIPCHK(e, hipMalloc(&e->ckoff, e->M * sizeof(int64_t)));
+        EHIPCHK(e, hipMalloc(&e->segoff, e->M * sizeof(int64_t)));
+        EHIPCHK(e, hipMemcpy(e->ckoff, ck.data(), e->M * sizeof(int64_t), hipMemcpyHostToDevice));
+        EHIPCHK(e, hipMemcpy(e->segoff, sg.data(), e->M * sizeof(int64_t), hipMemcpyHostToDevice));
+        e->ck_nsteps = nsteps;
+        e->ck_every = every;
+    }
+    const int *ord = e->order[e->order_kind];
+    const dim3 grid(e->M);
+    // lanes: 64 for members of at most 32 levels; otherwise 64 or 128 (the
+    // 256-lane forward schedule runs its adjoint with 128)
+#define CA_ADJ_GO(LPM, NF)                                                                                         \
+    k_ensemble_adjoint<LPM, NF><<<grid, LPM, 0, st>>>(e->geoms, e->n, ord, e->doff, Y0, YbN, Yb0, D0b, awb, e->ckpt, \
+                                                      e->ckoff, e->segb, e->segoff, e->tab, (float)dt, nsteps, every)
+    if (e->nf == 33) CA_ADJ_GO(64, 33);
+    else if (e->nf == 65 && e->lanes == 64) CA_ADJ_GO(64, 65);
+    else if (e->nf == 65) CA_ADJ_GO(128, 65);
+    else if (e->lanes == 64) CA_ADJ_GO(64, 129);
+    else CA_ADJ_GO(128, 129);
+#undef CA_ADJ_GO
+    EHIPCHK(e, hipGetLastError());
+    return CA_OK;
 }
 
 }  // extern "C"

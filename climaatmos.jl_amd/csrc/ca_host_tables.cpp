@@ -7,12 +7,22 @@
 
 extern "C" {
 
+void *ca_host_dss_build_t(int32_t nelem, int32_t ncol, int32_t ngroups, const int32_t *dss_ptr, const int32_t *dss_col,
+                          const double *dss_w, const double *basis, int32_t transpose, char *err, int32_t errlen);
+
 // Builds the DSS tables; returns an opaque pointer (NULL on failure, with
 // the reason in err).
 void *ca_host_dss_build(int32_t nelem, int32_t ncol, int32_t ngroups, const int32_t *dss_ptr, const int32_t *dss_col,
                         const double *dss_w, const double *basis, char *err, int32_t errlen) {
+    return ca_host_dss_build_t(nelem, ncol, ngroups, dss_ptr, dss_col, dss_w, basis, 0, err, errlen);
+}
+
+// As ca_host_dss_build; transpose != 0 builds the adjoint DSS's tables.
+void *ca_host_dss_build_t(int32_t nelem, int32_t ncol, int32_t ngroups, const int32_t *dss_ptr, const int32_t *dss_col,
+                          const double *dss_w, const double *basis, int32_t transpose, char *err, int32_t errlen) {
     DssTable *T = new DssTable();
-    const std::string msg = ca_build_dss_table(nelem, ncol, ngroups, dss_ptr, dss_col, dss_w, basis, *T);
+    const std::string msg =
+        ca_build_dss_table(nelem, ncol, ngroups, dss_ptr, dss_col, dss_w, basis, *T, transpose != 0);
     if (!msg.empty()) {
         if (err && errlen > 0) {
             std::strncpy(err, msg.c_str(), (size_t)errlen - 1);

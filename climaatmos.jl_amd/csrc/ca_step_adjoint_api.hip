@@ -1,0 +1,265 @@
+// ca_step_adjoint_api.hip -- host side of the sphere step's reverse mode:
+// ca_dss_vjp, ca_remaining_tendency_vjp and ca_step_adjoint.  Included after
+// ca_api.hip in the single translation unit (it uses the handle internals).
+#include "ca_kernels.h"
+
+namespace {
+
+// planes of the S-sized reverse-mode scratch h->aw_buf
+enum AdjBuf { AB_UB, AB_CB, AB_UTB1, AB_UTB2, AB_UTB3, AB_FEB, AB_FIB, AB_GB, AB_UBS, AB_Y, AB_COUNT };
+
+ca_status adj_ensure(ca_handle *h, int64_t rec_steps, int64_t nck) {
+    const DevGeom &g = h->g;
+    ca_status s;
+    if (!h->aw_buf) {
+        if ((s = dalloc(h, &h->aw_buf, (size_t)AB_COUNT * h->S))) return s;
+        if ((s = dalloc(h, &h->aw_L, 16 * (size_t)g.nC))) return s;
+        const int nch = (int)cdiv(g.nlev, h->adj_chunk);
+        if ((s = dalloc(h, &h->aw_d0, (size_t)nch * g.ncol))) return s;
+        // per-column one-column views of the geometry (the Newton stage reverse)
+        std::vector<DevGeom> v(g.ncol);
+        for (int c = 0; c < g.ncol; ++c) {
+            DevGeom cg = g;
+            cg.nelem = 0;
+            cg.ncol = 1;
+            cg.nC = g.nlev;
+            cg.nF = g.nlev + 1;
+            for (const float **p : {&cg.Jh, &cg.gu11, &cg.gu12, &cg.gu22, &cg.gd11, &cg.gd12, &cg.gd22, &cg.f3n,
+                                    &cg.f12a, &cg.f12b, &cg.iJh})
+                *p += c;
+            cg.basis = nullptr;
+            if (cg.aw_col) cg.aw_col += c;
+            if (cg.D0_col) cg.D0_col += c;
+            cg.gtc = nullptr;
+            cg.gtw = cg.gtM = nullptr;
+            cg.ncl = 0;
+            cg.dl_list = cg.dl_nE = cg.dl_col = nullptr;
+            cg.dl_slot = nullptr;
+            cg.dl_w = cg.dl_M = nullptr;
+            v[c] = cg;
+        }
+        if ((s = upload(h, &h->cgeo, v))) return s;
+    }
+    if (rec_steps > h->aw_rec_steps) {
+        if ((s = dalloc(h, &h->aw_rec, (size_t)(rec_steps * rec_size(h))))) return s;
+        h->aw_rec_steps = rec_steps;
+    }
+    if (nck > h->aw_ck_n) {
+        if ((s = dalloc(h, &h->aw_ck, (size_t)(nck * h->S)))) return s;
+        h->aw_ck_n = nck;
+    }
+    return CA_OK;
+}
+
+float *abuf(ca_handle *h, int i) { return h->aw_buf + (int64_t)i * h->S; }
+
+// DSS^T of a state-shaped buffer, src -> dst (filter: also the u_3 boundary filter)
+ca_status dss_T_state(ca_handle *h, const float *src, float *dst, int filter, hipStream_t st) {
+    const DevGeom &g = h->gT;
+    const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
+    if (h->dss_kind == 0 && g.ncl > 0)
+        k_dss_lds<4, 1, true><<<dss_lds_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+    else
+        k_dss_gather<4, 1, true><<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+    return check_launch(h, "k_dss(transposed state)");
+}
+
+ca_status lincomb(ca_handle *h, float *out, std::initializer_list<std::pair<const float *, double>> terms, bool acc,
+                  unsigned planes, hipStream_t st) {
+    sadj::Lin L{};
+    L.nx = 0;
+    for (const auto &t : terms) {
+        if (t.second == 0.0) continue;
+        if (L.nx >= 6) return fail(h, CA_ERR_ARG, "lincomb: too many terms");
+        L.x[L.nx] = t.first;
+        L.a[L.nx] = (float)t.second;
+        ++L.nx;
+    }
+    L.acc = acc ? 1 : 0;
+    L.planes = planes;
+    sadj::k_lincomb<<<cdiv(h->S, 256), 256, 0, st>>>(h->g, out, L);
+    return check_launch(h, "k_lincomb");
+}
+
+constexpr unsigned PL_ALL = 0x1F, PL_FI = (1u << 0) | (1u << 3) | (1u << 4);  // rho, rho e_tot, u_3
+
+// Yb = (d T_E / d U)^T gb at U with the DSSed Laplacians L (hyperdiffusion);
+// D0 cotangents are added to h->aw_d0.  gb: cotangent of the un-DSSed
+// explicit tendency.
+ca_status explicit_vjp(ca_handle *h, const float *U, const float *L, const float *gb, float *Yb, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const DevNum &n = h->n;
+    const int ch = h->adj_chunk;
+    const dim3 grid(cdiv(g.ncol, 256), cdiv(g.nlev, ch)), blk(256);
+    float *Lb = h->aw_L, *Lb2 = h->aw_L + 4 * g.nC, *fa = h->aw_L + 8 * g.nC, *fb = h->aw_L + 12 * g.nC;
+    const bool hy = n.hyper != 0, vd = n.vdiff != 0;
+    if (hy && vd) sadj::k_explicit_vjp<true, true><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
+    else if (hy) sadj::k_explicit_vjp<true, false><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
+    else if (vd) sadj::k_explicit_vjp<false, true><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
+    else sadj::k_explicit_vjp<false, false><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
+    ca_status s = check_launch(h, "k_explicit_vjp");
+    if (s || !hy) return s;
+    {
+        const DevGeom &gT = h->gT;
+        const int dch = h->dss_chunk > 0 ? h->dss_chunk : gT.nlev;
+        if (h->dss_kind == 0 && gT.ncl > 0)
+            k_dss_lds<4, 0, false><<<dss_lds_grid(gT, dch), 256, 0, st>>>(gT, Lb, Lb2, 0, dch);
+        else
+            k_dss_gather<4, 0, false><<<dss_grid(gT, dch), 256, 0, st>>>(gT, Lb, Lb2, 0, dch);
+    }
+    if ((s = check_launch(h, "k_dss(transposed laplacians)"))) return s;
+    sadj::k_laplacians_vjp<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, n, U, Lb2, Yb, fa, fb);
+    if ((s = check_launch(h, "k_laplacians_vjp"))) return s;
+    if (g.nlev >= 2) {
+        sadj::k_fold_faces<<<dim3(cdiv(g.ncol, 256), g.nlev - 1), 256, 0, st>>>(g, Yb, fa, fb);
+        s = check_launch(h, "k_fold_faces");
+    }
+    return s;
+}
+
+// Utb = (d Newton_i / d U~)^T Ub at U~ = Ut, column by column
+ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Ub, float dtg, float *Utb, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const dim3 grid(g.ncol);
+    if (g.nlev <= 32)
+        sadj::k_newton_vjp<64, 33><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb);
+    else if (g.nlev <= 64)
+        sadj::k_newton_vjp<64, 65><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb);
+    else
+        sadj::k_newton_vjp<128, 129><<<grid, 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb);
+    return check_launch(h, "k_newton_vjp");
+}
+
+ca_status zero_d0(ca_handle *h, hipStream_t st) {
+    const int nch = (int)cdiv(h->g.nlev, h->adj_chunk);
+    HIPCHK(h, hipMemsetAsync(h->aw_d0, 0, (size_t)nch * h->g.ncol * sizeof(double), st));
+    return CA_OK;
+}
+
+ca_status sum_d0(ca_handle *h, float *D0b, hipStream_t st) {
+    const int nch = (int)cdiv(h->g.nlev, h->adj_chunk);
+    sadj::k_d0_sum<<<cdiv(h->g.ncol, 256), 256, 0, st>>>(h->g.ncol, nch, h->aw_d0, D0b);
+    return check_launch(h, "k_d0_sum");
+}
+
+// Reverse of one step (one_step with records rec): on entry AB_UB holds the
+// cotangent of the step's result, on exit that of its start state.
+ca_status reverse_step(ca_handle *h, float *rec, double dt, hipStream_t st) {
+    ca_status s;
+    float *ub = abuf(h, AB_UB), *Cb = abuf(h, AB_CB), *feb = abuf(h, AB_FEB), *fib = abuf(h, AB_FIB);
+    float *gb = abuf(h, AB_GB), *Ubs = abuf(h, AB_UBS);
+    float *Utb[4] = {nullptr, abuf(h, AB_UTB1), abuf(h, AB_UTB2), abuf(h, AB_UTB3)};
+    // u' = DSS_f(C), C = u + dt sum_j b_j (F_E_j + F_I_j)
+    if ((s = dss_T_state(h, ub, Cb, 1, st))) return s;
+    for (int i = 3; i >= 1; --i) {
+        const double dtg = dt * h->aI[i][i];
+        // cotangents of F_E_i and F_I_i: from C and from the later stages' U~
+        auto later = [&](double (*a)[4], int l) { return l > i ? dt * a[l][i] : 0.0; };
+        if ((s = lincomb(h, feb, {{Cb, dt * h->b[i]}, {Utb[1], later(h->aE, 1)}, {Utb[2], later(h->aE, 2)},
+                                  {Utb[3], later(h->aE, 3)}},
+                         false, PL_ALL, st)))
+            return s;
+        if ((s = lincomb(h, fib, {{Cb, dt * h->b[i]}, {Utb[1], later(h->aI, 1)}, {Utb[2], later(h->aI, 2)},
+                                  {Utb[3], later(h->aI, 3)}},
+                         false, PL_FI, st)))
+            return s;
+        // F_E_i = DSS(T_E(U_i))
+        if ((s = dss_T_state(h, feb, gb, 0, st))) return s;
+        if ((s = explicit_vjp(h, rec_U(h, rec, i), rec_L(h, rec, i), gb, Ubs, st))) return s;
+        // F_I_i = (U_i - U~_i) / dtg
+        if ((s = lincomb(h, Ubs, {{fib, 1.0 / dtg}}, true, PL_FI, st))) return s;
+        // U_i = Newton(U~_i)
+        if ((s = newton_vjp(h, rec_Ut(h, rec, i), Ubs, (float)dtg, Utb[i], st))) return s;
+        if ((s = lincomb(h, Utb[i], {{fib, -1.0 / dtg}}, true, PL_FI, st))) return s;
+    }
+    // F_E_0 = DSS(T_E(u))
+    if ((s = lincomb(h, feb, {{Cb, dt * h->b[0]}, {Utb[1], dt * h->aE[1][0]}, {Utb[2], dt * h->aE[2][0]},
+                              {Utb[3], dt * h->aE[3][0]}},
+                     false, PL_ALL, st)))
+        return s;
+    if ((s = dss_T_state(h, feb, gb, 0, st))) return s;
+    if ((s = explicit_vjp(h, rec_U(h, rec, 0), rec_L(h, rec, 0), gb, Ubs, st))) return s;
+    // u enters C and every U~_i with weight 1
+    return lincomb(h, ub, {{Cb, 1.0}, {Utb[1], 1.0}, {Utb[2], 1.0}, {Utb[3], 1.0}, {Ubs, 1.0}}, false, PL_ALL, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream) {
+    if (!h || !Yb) return CA_ERR_ARG;
+    if (h->g.nelem == 0) return CA_OK;
+    hipStream_t st = (hipStream_t)stream;
+    ca_status s;
+    if ((s = adj_ensure(h, 0, 0))) return s;
+    float *tmp = abuf(h, AB_GB);
+    HIPCHK(h, hipMemcpyAsync(tmp, Yb, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return dss_T_state(h, tmp, Yb, 0, st);
+}
+
+ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, const float *Y, float *D0b,
+                                    void *stream) {
+    if (!h || !Yb || !Ytb || !Y) return CA_ERR_ARG;
+    if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: sphere grids only (columns: ca_ensemble_adjoint)");
+    hipStream_t st = (hipStream_t)stream;
+    const DevGeom &g = h->g;
+    ca_status s;
+    if ((s = adj_ensure(h, 0, 0))) return s;
+    if (h->n.hyper) {  // the DSSed Laplacians at Y
+        {
+            Launch l(h, st, KID_LAPLACIANS);
+            k_laplacians<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, h->n, Y, h->raw);
+        }
+        {
+            Launch l(h, st, KID_DSS_LAPLACIAN);
+            launch_dss<4, 0, false>(h, h->raw, h->L, 0, st);
+        }
+        if ((s = check_launch(h, "laplacians"))) return s;
+    }
+    if ((s = zero_d0(h, st))) return s;
+    if ((s = explicit_vjp(h, Y, h->L, Ytb, Yb, st))) return s;
+    if (D0b && h->n.vdiff) return sum_d0(h, D0b, st);
+    return CA_OK;
+}
+
+ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, double dt, int32_t nsteps, int32_t every,
+                          float *Yb0, float *D0b, void *stream) {
+    if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t S = h->S;
+    const int nseg = (nsteps + every - 1) / every;
+    ca_status s;
+    if ((s = adj_ensure(h, every, nseg))) return s;
+    float *Yw = abuf(h, AB_Y);
+    // forward: the step's initial dss! and filter, then checkpoints every `every` steps
+    HIPCHK(h, hipMemcpyAsync(h->U, Y0, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if ((s = dss_gather_state(h, h->U, Yw, 1, st, KID_DSS_STATE))) return s;
+    HIPCHK(h, hipMemcpyAsync(h->aw_ck, Yw, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    for (int t = 0; t < (nseg - 1) * every; ++t) {
+        if ((s = one_step(h, Yw, dt, st, nullptr))) return s;
+        if ((t + 1) % every == 0)
+            HIPCHK(h, hipMemcpyAsync(h->aw_ck + (int64_t)((t + 1) / every) * S, Yw, (size_t)S * sizeof(float),
+                                     hipMemcpyDeviceToDevice, st));
+    }
+    // reverse: per segment (last first), recompute with records, then reverse its steps
+    if ((s = zero_d0(h, st))) return s;
+    HIPCHK(h, hipMemcpyAsync(abuf(h, AB_UB), YbN, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    const int64_t RS = rec_size(h);
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int len = std::min(every, nsteps - sg * every);
+        HIPCHK(h, hipMemcpyAsync(Yw, h->aw_ck + (int64_t)sg * S, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice,
+                                 st));
+        for (int j = 0; j < len; ++j)
+            if ((s = one_step(h, Yw, dt, st, h->aw_rec + j * RS))) return s;
+        for (int j = len - 1; j >= 0; --j)
+            if ((s = reverse_step(h, h->aw_rec + j * RS, dt, st))) return s;
+    }
+    // the initial dss! and filter
+    if ((s = dss_T_state(h, abuf(h, AB_UB), Yb0, 1, st))) return s;
+    if (D0b && h->n.vdiff) return sum_d0(h, D0b, st);
+    return CA_OK;
+}
+
+}  // extern "C"

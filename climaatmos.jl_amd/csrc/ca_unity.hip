@@ -5,6 +5,9 @@
 #include "ca_spectral.hip"
 #include "ca_dss.hip"
 #include "ca_ensemble.hip"
+#include "ca_adjoint.hip"
+#include "ca_step_adjoint.hip"
 #include "ca_dss_table.cpp"
 #include "ca_api.hip"
 #include "ca_ensemble_api.hip"
+#include "ca_step_adjoint_api.hip"

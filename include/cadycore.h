@@ -202,6 +202,43 @@ ca_status ca_ensemble_get_tuning(const ca_ensemble *e, int32_t knob, int32_t *va
  * 0/1 in that order */
 ca_status ca_ensemble_autotune(ca_ensemble *e, const float *Y, double dt, int32_t nsteps, void *stream,
                                double *times_ms);
+/* new per-member alpha_w and / or D0 (host arrays of nmembers; NULL keeps) */
+ca_status ca_ensemble_set_params(ca_ensemble *e, const double *alpha_w, const double *D0);
+/* Reverse mode of nsteps ensemble steps (the vector-Jacobian product of the
+ * solution operator), with step checkpointing every `every` steps.  No
+ * reference counterpart: the reference differentiates only its Jacobians, in
+ * forward mode (implicit/autodiff_utils.jl), and calibrates without gradients.
+ * Given the initial states Y0 and the cotangent YbN of the states after
+ * nsteps steps, writes the cotangent of Y0 to Yb0 (ragged, like Y0) and of
+ * each member's D0 and alpha_w to D0b, awb (device arrays of nmembers).
+ * Recomputes the forward trajectory itself; runs with 64 lanes per member
+ * when every member has at most 32 levels or CA_ENS_LANES is 64, else 128. */
+ca_status ca_ensemble_adjoint(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int32_t nsteps,
+                              int32_t every, float *Yb0, float *D0b, float *awb, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Reverse mode of the sphere step (sphere grids; independent columns use
+ * ca_ensemble_adjoint).  No reference counterpart: the reference has no
+ * adjoint (forward-mode AD for its Jacobians only, implicit/autodiff_utils.jl;
+ * gradient-free calibration in calibration/).  These are the pullbacks a
+ * ChainRulesCore rrule or a torch.autograd.Function binds.  Device pointers;
+ * the cotangents have the state's layout.
+ * ------------------------------------------------------------------------ */
+/* Yb <- DSS^T Yb, in place: the transpose of ca_dss (dss!, constrain_state.jl:59) */
+ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream);
+/* Yb <- (d T_exp / d Y)^T Ytb at Y: the pullback of ca_remaining_tendency
+ * (remaining_tendency!, remaining_tendency.jl:48), Laplacian DSS included.
+ * With vertical diffusion and D0b != NULL, D0b [ncol] <- d <Ytb, T_exp> / d D0. */
+ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, const float *Y, float *D0b,
+                                    void *stream);
+/* The pullback of ca_step(h, Y0, dt, nsteps): given the cotangent YbN of the
+ * state after nsteps steps from Y0, writes the cotangent of Y0 to Yb0 and,
+ * with vertical diffusion and D0b != NULL, the cotangent of each column's D0
+ * to D0b [ncol].  The forward trajectory is recomputed: checkpoints every
+ * `every` steps, and each segment is re-run once with its stage states
+ * recorded (device memory: (ceil(nsteps / every) + every * 8) states). */
+ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, double dt, int32_t nsteps,
+                          int32_t every, float *Yb0, float *D0b, void *stream);
 
 #ifdef __cplusplus
 }

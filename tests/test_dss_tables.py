@@ -23,20 +23,23 @@ def lib():
     L.ca_host_dss_build.restype = C.c_void_p
     L.ca_host_dss_build.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_char_p, C.c_int32]
+    L.ca_host_dss_build_t.restype = C.c_void_p
+    L.ca_host_dss_build_t.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
     L.ca_host_dss_nclusters.argtypes = [C.c_void_p]
     L.ca_host_dss_get.argtypes = [C.c_void_p] + [C.c_void_p] * 9
     L.ca_host_dss_free.argtypes = [C.c_void_p]
     return L
 
 
-def build(L, g):
+def build(L, g, transpose=False):
     ptr = np.ascontiguousarray(g.dss_ptr, np.int32)
     col = np.ascontiguousarray(g.dss_col, np.int32)
     w = np.ascontiguousarray(g.dss_w, np.float64)
     basis = np.ascontiguousarray(g.basis, np.float64)
     err = C.create_string_buffer(256)
-    h = L.ca_host_dss_build(g.nelem, g.ncol, len(ptr) - 1, ptr.ctypes.data, col.ctypes.data, w.ctypes.data,
-                            basis.ctypes.data, err, 256)
+    h = L.ca_host_dss_build_t(g.nelem, g.ncol, len(ptr) - 1, ptr.ctypes.data, col.ctypes.data, w.ctypes.data,
+                              basis.ctypes.data, int(transpose), err, 256)
     assert h, err.value
     ncl = L.ca_host_dss_nclusters(h)
     emax = L.ca_host_dss_emax()
@@ -113,3 +116,40 @@ def test_bad_groups_are_rejected(lib):
     h = lib.ca_host_dss_build(g.nelem, g.ncol, len(ptr) - 1, ptr.ctypes.data, col.ctypes.data, w.ctypes.data,
                               basis.ctypes.data, err, 256)
     assert not h and b"partition" in err.value
+
+
+def _apply(T, nc, x, v):
+    """The gather DSS the kernels perform, from the tables: scalar x [nc],
+    covariant vector v [nc, 2]."""
+    tc, tw, tM = T["tc"].reshape(nc, 4), T["tw"].reshape(nc, 4).astype(np.float64), T["tM"].reshape(nc, 4, 2, 2)
+    y = np.einsum("ct,ct->c", tw, x[tc])
+    u = np.einsum("ctij,ctj->ci", tM.astype(np.float64), v[tc])
+    return y, u
+
+
+@pytest.mark.parametrize("h_elem", [2, 3])
+def test_transposed_tables_are_the_adjoint_dss(lib, h_elem):
+    """<DSS x, y> = <x, DSS^T y> for scalars and covariant vectors, and the
+    transposed tables share the forward tables' slots and clusters."""
+    g = G.cubed_sphere(h_elem, G.uniform_faces(4, 30000.0), PAR.planet_radius, PAR.Omega)
+    F, B = build(lib, g), build(lib, g, transpose=True)
+    nc = g.ncol
+    for k in ("tc", "list", "nE", "ccol", "slot"):
+        np.testing.assert_array_equal(F[k], B[k])
+    rng = np.random.default_rng(h_elem)
+    x, y = rng.standard_normal(nc), rng.standard_normal(nc)
+    v, w = rng.standard_normal((nc, 2)), rng.standard_normal((nc, 2))
+    Fx, Fv = _apply(F, nc, x, v)
+    By, Bw = _apply(B, nc, y, w)
+    assert abs(Fx @ y - x @ By) <= 1e-6 * np.sqrt((Fx @ Fx) * (y @ y))
+    assert abs(np.sum(Fv * w) - np.sum(v * Bw)) <= 1e-6 * np.sqrt(np.sum(Fv * Fv) * np.sum(w * w))
+    # and the forward DSS is a projection: applying it twice changes nothing (float32 tables)
+    Fx2, Fv2 = _apply(F, nc, Fx, Fv)
+    np.testing.assert_allclose(Fx2, Fx, atol=1e-5)
+    np.testing.assert_allclose(Fv2, Fv, atol=1e-5)
+    # cluster copies of the transposed weights and transforms
+    ncl = F["ncl"]
+    ccol = F["ccol"].reshape(ncl, 256)
+    act = ccol >= 0
+    np.testing.assert_array_equal(B["cw"].reshape(ncl, 256, 4)[act], B["tw"].reshape(nc, 4)[ccol[act]])
+    np.testing.assert_array_equal(B["cM"].reshape(ncl, 256, 16)[act], B["tM"].reshape(nc, 16)[ccol[act]])

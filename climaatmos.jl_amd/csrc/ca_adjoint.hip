@@ -29,195 +29,177 @@
 namespace adj {
 
 
-// ------------------------------------------------------------ dual numbers
-template <int N>
-struct Dual {
-    double v;
-    double d[N];
-};
-template <int N>
-__device__ __forceinline__ Dual<N> dcon(double v) {
-    Dual<N> r;
-    r.v = v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = 0.0;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> dvar(double v, int i) {
-    Dual<N> r = dcon<N>(v);
-    r.d[i] = 1.0;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator+(const Dual<N> &a, const Dual<N> &b) {
-    Dual<N> r;
-    r.v = a.v + b.v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a, const Dual<N> &b) {
-    Dual<N> r;
-    r.v = a.v - b.v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a) {
-    Dual<N> r;
-    r.v = -a.v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator*(const Dual<N> &a, const Dual<N> &b) {
-    Dual<N> r;
-    r.v = a.v * b.v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator*(double s, const Dual<N> &a) {
-    Dual<N> r;
-    r.v = s * a.v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = s * a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator+(double s, const Dual<N> &a) {
-    Dual<N> r = a;
-    r.v += s;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator+(const Dual<N> &a, double s) {
-    Dual<N> r = a;
-    r.v += s;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a, double s) {
-    Dual<N> r = a;
-    r.v -= s;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator-(double s, const Dual<N> &a) {
-    Dual<N> r;
-    r.v = s - a.v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> drcp(const Dual<N> &a) {
-    const double iv = 1.0 / a.v, g = -iv * iv;
-    Dual<N> r;
-    r.v = iv;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = g * a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> dapply(const Dual<N> &a, double v, double slope) {
-    Dual<N> r;
-    r.v = v;
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.d[i] = slope * a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> dlog(const Dual<N> &a) { return dapply(a, log(a.v), 1.0 / a.v); }
-template <int N>
-__device__ __forceinline__ Dual<N> dexp(const Dual<N> &a) {
-    const double e = exp(a.v);
-    return dapply(a, e, e);
-}
-template <int N>
-__device__ __forceinline__ Dual<N> dlog1p(const Dual<N> &a) { return dapply(a, log1p(a.v), 1.0 / (1.0 + a.v)); }
-template <int N>
-__device__ __forceinline__ Dual<N> dexpm1(const Dual<N> &a) { return dapply(a, expm1(a.v), exp(a.v)); }
-template <int N>
-__device__ __forceinline__ Dual<N> dpow7(const Dual<N> &a) {
-    const double a2 = a.v * a.v, a6 = a2 * a2 * a2;
-    return dapply(a, a6 * a.v, 7.0 * a6);
-}
-
-// ------------------------------------------------- face row in dual numbers
-// Inputs (the 11 values a face row of the Newton stage reads): rho, u1, u2,
-// rho e_tot of the centres below (a) and above (b) the face, and u_3 on the
-// faces f - 1, f, f + 1.  Outputs, as in jac_face_row and pgf_u3: lo, d, up,
+// ------------------------------------------------------- face row reverse
+// The 11 values a face row of the Newton stage reads: rho, u1, u2, rho e_tot
+// of the centres below (a) and above (b) the face, and u_3 on the faces
+// f - 1, f, f + 1.  Its outputs, as in jac_face_row and pgf_u3: lo, d, up,
 // r0, r1, e0, e1 of the Schur row and Ru3 = dtg (pgf - beta u3).
 constexpr int NP = 11;
-using D11 = Dual<NP>;
-struct CVD {
-    D11 rho, K, T, p, Pi;
-};
 
-__device__ __forceinline__ CVD cv_dual(const DevGeom &g, const DevNum &n, int k, const D11 &rho, const D11 &u1,
-                                       const D11 &u2, const D11 &rhoe, const D11 &u3a, const D11 &u3b) {
-    CVD c;
+// thermodynamic state of one centre in float64 (cv_make's arithmetic)
+struct Cen {
+    double rho, u1, u2, ct1, ct2, re, e, K, T, p, Pi;
+    bool free_T;
+};
+__device__ __forceinline__ Cen cen_eval(const DevGeom &g, const DevNum &n, int k, double rho, double u1, double u2,
+                                        double re, double u3a, double u3b) {
+    Cen c;
     c.rho = rho;
+    c.u1 = u1;
+    c.u2 = u2;
+    c.re = re;
     const double is2 = g.is2c[k];
-    const D11 ct1 = is2 * ((double)g.gu11[0] * u1 + (double)g.gu12[0] * u2);
-    const D11 ct2 = is2 * ((double)g.gu12[0] * u1 + (double)g.gu22[0] * u2);
-    const double ga = g.g33f[k], gb = g.g33f[k + 1];
-    c.K = 0.5 * ((u1 * ct1 + u2 * ct2) + 0.5 * (ga * (u3a * u3a) + gb * (u3b * u3b)));
-    const D11 e_tot = rhoe * drcp(rho);
-    D11 T = n.dT_0 + n.dicv_d * (e_tot - c.K + (-g.Phic_d[k] + n.dR_d * n.dT_0));
-    if (!(T.v > n.dTmin)) T = dcon<NP>(n.dTmin);
-    c.T = T;
-    c.p = n.dR_d * (rho * T);
-    c.Pi = dexp(n.dkex * dlog(n.dp0inv * c.p));
+    c.ct1 = is2 * ((double)g.gu11[0] * u1 + (double)g.gu12[0] * u2);
+    c.ct2 = is2 * ((double)g.gu12[0] * u1 + (double)g.gu22[0] * u2);
+    c.K = 0.5 * ((u1 * c.ct1 + u2 * c.ct2) + 0.5 * ((double)g.g33f[k] * (u3a * u3a) + (double)g.g33f[k + 1] * (u3b * u3b)));
+    c.e = re / rho;
+    const double Tu = n.dT_0 + n.dicv_d * (c.e - c.K + (-g.Phic_d[k] + n.dR_d * n.dT_0));
+    c.free_T = Tu > n.dTmin;
+    c.T = c.free_T ? Tu : n.dTmin;
+    c.p = n.dR_d * (rho * c.T);
+    c.Pi = exp(n.dkex * log(n.dp0inv * c.p));
     return c;
 }
+// reverse of cen_eval: cotangents of T, p (direct) and Pi -> rho, u1, u2, rho e_tot, u3a, u3b
+__device__ __forceinline__ void cen_reverse(const DevGeom &g, const DevNum &n, int k, const Cen &c, double u3a,
+                                            double u3b, double Tb, double pb, double Pib, double Kb, double *out) {
+    pb += Pib * n.dkex * c.Pi / c.p;
+    double rhob = n.dR_d * c.T * pb;
+    Tb += n.dR_d * c.rho * pb;
+    const double Tub = c.free_T ? Tb : 0.0;
+    const double eb = n.dicv_d * Tub;
+    Kb -= n.dicv_d * Tub;
+    out[3] += eb / c.rho;
+    rhob -= eb * c.e / c.rho;
+    out[0] += rhob;
+    // K = (u . ct) / 2 + (g_a u3a^2 + g_b u3b^2) / 4, ct = is2 G^-1 u (symmetric)
+    out[1] += Kb * c.ct1;
+    out[2] += Kb * c.ct2;
+    out[4] += Kb * 0.5 * (double)g.g33f[k] * u3a;
+    out[5] += Kb * 0.5 * (double)g.g33f[k + 1] * u3b;
+}
 
-struct FaceOut {
-    D11 lo, d, up, r0, r1, e0, e1, Ru3;
+struct FaceVals {
+    double r0, r1, e0, e1;
 };
 
-__device__ __forceinline__ FaceOut face_dual(const DevGeom &g, const DevNum &n, int f, float dtgf, const float *prim) {
-    D11 x[NP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) x[i] = dvar<NP>((double)prim[i], i);
+// Face row f forward (float64) and its reverse seeded by ob = the
+// cotangents of (lo, d, up, r0, r1, e0, e1, Ru3): pb receives the
+// cotangents of the 11 inputs.  Returns the row's r0, r1, e0, e1.
+__device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, int f, float dtgf, const float *prim,
+                                             const double *ob, double *pb) {
     const double dtg = dtgf;
-    const CVD a = cv_dual(g, n, f - 1, x[0], x[1], x[2], x[3], x[8], x[9]);
-    const CVD b = cv_dual(g, n, f, x[4], x[5], x[6], x[7], x[9], x[10]);
-    const double kap = n.dR_d / n.dcv_d, cp = n.dcp_d;
-    FaceOut o;
-    const D11 rf = 0.5 * (a.rho + b.rho);
-    const D11 pg_m = drcp(rf), pg_p = -pg_m;
-    const D11 dpa = kap * ((n.dT_0 * cp - (double)g.Phic[f - 1]) + (-a.K));
-    const D11 dpb = kap * ((n.dT_0 * cp - (double)g.Phic[f]) + (-b.K));
-    const D11 thva = a.T * drcp(a.Pi), thvb = b.T * drcp(b.Pi);
-    const D11 lr = n.dkex * dlog1p((b.p - a.p) * drcp(a.p));
-    const D11 dPi = a.Pi * dexpm1(lr);
-    const D11 buoy = cp * ((0.5 * (thva + thvb)) * dPi * pg_m);
-    o.r0 = dtg * (pg_m * dpa + 0.5 * buoy);
-    o.r1 = dtg * (pg_p * dpb + 0.5 * buoy);
-    o.e0 = (dtg * kap) * pg_m;
-    o.e1 = (dtg * kap) * pg_p;
-    const D11 ka = (-kap) * a.rho, kb = (-kap) * b.rho;
+    const double x8 = prim[8], x9 = prim[9], x10 = prim[10];
+    const Cen a = cen_eval(g, n, f - 1, prim[0], prim[1], prim[2], prim[3], x8, x9);
+    const Cen b = cen_eval(g, n, f, prim[4], prim[5], prim[6], prim[7], x9, x10);
+    const double kap = n.dR_d / n.dcv_d, cp = n.dcp_d, dTs = n.dTsr - n.dTmr;
+    // ---- forward
+    const double rf = 0.5 * (a.rho + b.rho);
+    const double pgm = 1.0 / rf, pgp = -pgm;
+    const double dpa = kap * ((n.dT_0 * cp - (double)g.Phic[f - 1]) - a.K);
+    const double dpb = kap * ((n.dT_0 * cp - (double)g.Phic[f]) - b.K);
+    const double thva = a.T / a.Pi, thvb = b.T / b.Pi;
+    const double z = (b.p - a.p) / a.p;
+    const double lr = n.dkex * log1p(z);
+    const double Em = expm1(lr);
+    const double dPi = a.Pi * Em;
+    const double thbar = 0.5 * (thva + thvb);
+    const double buoy = cp * thbar * dPi * pgm;
+    FaceVals v;
+    v.r0 = dtg * (pgm * dpa + 0.5 * buoy);
+    v.r1 = dtg * (pgp * dpb + 0.5 * buoy);
+    v.e0 = dtg * kap * pgm;
+    v.e1 = dtg * kap * pgp;
+    const double ka = -kap * a.rho, kb = -kap * b.rho;
     const double gm = g.g33f[f - 1], g0 = g.g33f[f], gp = g.g33f[f + 1];
     const double beta = n.rayleigh ? (double)g.aw_col[0] * (double)g.zeta_f[f] : 0.0;
-    o.lo = dtg * (pg_m * ka * ((0.5 * gm) * x[8]));
-    o.d = -1.0 + dtg * ((pg_m * ka + pg_p * kb) * ((0.5 * g0) * x[9]) + dcon<NP>(-beta));
-    o.up = dtg * (pg_p * kb * ((0.5 * gp) * x[10]));
-    // pgf_u3 (cancellation-free reference-state differences)
-    const D11 Pa = a.Pi, Pa7 = dpow7(Pa);
-    const D11 dPhr = (-cp) * (n.dTmr * lr + ((n.dTsr - n.dTmr) / 7.0) * (Pa7 * dexpm1(7.0 * lr)));
-    const D11 Pb = Pa + dPi, Pb7 = dpow7(Pb);
-    const D11 tha = (a.T - (n.dTmr + (n.dTsr - n.dTmr) * Pa7)) * drcp(Pa);
-    const D11 thb = (b.T - (n.dTmr + (n.dTsr - n.dTmr) * Pb7)) * drcp(Pb);
-    const D11 pgf = -((g.dPhif_d[f] + (-dPhr)) + cp * ((0.5 * (tha + thb)) * dPi));
-    o.Ru3 = dtg * (pgf + (-beta) * x[9]);
-    return o;
+    const double Pa7 = a.Pi * a.Pi * a.Pi * a.Pi * a.Pi * a.Pi * a.Pi;
+    const double E7 = expm1(7.0 * lr);
+    const double Pb = a.Pi + dPi;
+    const double Pb7 = Pb * Pb * Pb * Pb * Pb * Pb * Pb;
+    const double tha = (a.T - (n.dTmr + dTs * Pa7)) / a.Pi;
+    const double thb = (b.T - (n.dTmr + dTs * Pb7)) / Pb;
+    // ---- reverse
+    const double Lb = ob[0], Db = ob[1], Ub = ob[2], r0b = ob[3], r1b = ob[4], e0b = ob[5], e1b = ob[6], qb = ob[7];
+    double x8b = 0.0, x9b = 0.0, x10b = 0.0;
+    // Ru3 = dtg (pgf - beta u3); pgf = -((dPhi - dPhr) + cp (tha + thb)/2 dPi)
+    const double pgfb = dtg * qb;
+    x9b -= dtg * beta * qb;
+    const double dPhrb = pgfb;
+    const double thab = -0.5 * cp * dPi * pgfb, thbb = thab;
+    double dPib = -cp * 0.5 * (tha + thb) * pgfb;
+    // thb = (T_b - T_r(Pb)) / Pb, tha likewise with Pa
+    double Tbb = thbb / Pb, Tab = thab / a.Pi;
+    double Pbb = -thbb * thb / Pb - thbb * dTs * 7.0 * (Pb7 / Pb) / Pb;
+    double Pab = -thab * tha / a.Pi;
+    double Pa7b = -dTs * thab / a.Pi;
+    Pab += Pbb;
+    dPib += Pbb;
+    // dPhr = -cp (T_min lr + dTs / 7 Pa^7 expm1(7 lr))
+    double lrb = -cp * n.dTmr * dPhrb;
+    Pa7b += -cp * dTs / 7.0 * E7 * dPhrb;
+    lrb += -cp * dTs / 7.0 * Pa7 * 7.0 * (E7 + 1.0) * dPhrb;
+    Pab += 7.0 * (Pa7 / a.Pi) * Pa7b;
+    // up, d, lo
+    double pgmb = 0.0, pgpb = 0.0, kab = 0.0, kbb = 0.0;
+    pgpb += dtg * kb * 0.5 * gp * x10 * Ub;
+    kbb += dtg * pgp * 0.5 * gp * x10 * Ub;
+    x10b += dtg * pgp * kb * 0.5 * gp * Ub;
+    const double c9 = 0.5 * g0 * x9;
+    pgmb += dtg * ka * c9 * Db;
+    kab += dtg * pgm * c9 * Db;
+    pgpb += dtg * kb * c9 * Db;
+    kbb += dtg * pgp * c9 * Db;
+    x9b += dtg * (pgm * ka + pgp * kb) * 0.5 * g0 * Db;
+    pgmb += dtg * ka * 0.5 * gm * x8 * Lb;
+    kab += dtg * pgm * 0.5 * gm * x8 * Lb;
+    x8b += dtg * pgm * ka * 0.5 * gm * Lb;
+    double rab = -kap * kab, rbb = -kap * kbb;
+    // e0, e1, r0, r1
+    pgmb += dtg * kap * e0b;
+    pgpb += dtg * kap * e1b;
+    pgpb += dtg * dpb * r1b;
+    const double dpbb = dtg * pgp * r1b;
+    pgmb += dtg * dpa * r0b;
+    const double dpab = dtg * pgm * r0b;
+    const double buoyb = 0.5 * dtg * (r0b + r1b);
+    // buoy = cp thbar dPi pgm
+    const double thbarb = cp * dPi * pgm * buoyb;
+    dPib += cp * thbar * pgm * buoyb;
+    pgmb += cp * thbar * dPi * buoyb;
+    // dPi = Pa expm1(lr), lr = kex log1p((pb - pa) / pa)
+    Pab += Em * dPib;
+    lrb += a.Pi * (Em + 1.0) * dPib;
+    const double zb = n.dkex * lrb / (1.0 + z);
+    double pbb = zb / a.p;
+    double pab = -zb * b.p / (a.p * a.p);
+    // thv = T / Pi
+    Tab += 0.5 * thbarb / a.Pi;
+    Pab -= 0.5 * thbarb * thva / a.Pi;
+    Tbb += 0.5 * thbarb / b.Pi;
+    double Pib_b = -0.5 * thbarb * thvb / b.Pi;
+    // dp = kap ((T0 cp - Phi) - K)
+    const double Kab = -kap * dpab, Kbb = -kap * dpbb;
+    // pgp = -pgm, pgm = 1 / rf, rf = (rho_a + rho_b) / 2
+    pgmb -= pgpb;
+    const double rfb = -pgm * pgm * pgmb;
+    rab += 0.5 * rfb;
+    rbb += 0.5 * rfb;
+    double oa[6] = {rab, 0.0, 0.0, 0.0, 0.0, 0.0}, obb[6] = {rbb, 0.0, 0.0, 0.0, 0.0, 0.0};
+    cen_reverse(g, n, f - 1, a, x8, x9, Tab, pab, Pab, Kab, oa);
+    cen_reverse(g, n, f, b, x9, x10, Tbb, pbb, Pib_b, Kbb, obb);
+    pb[0] = oa[0];
+    pb[1] = oa[1];
+    pb[2] = oa[2];
+    pb[3] = oa[3];
+    pb[4] = obb[0];
+    pb[5] = obb[1];
+    pb[6] = obb[2];
+    pb[7] = obb[3];
+    pb[8] = x8b + oa[4];
+    pb[9] = x9b + oa[5] + obb[4];
+    pb[10] = x10b + obb[5];
+    return v;
 }
 
 // --------------------------------------------------------------- LDS layout
@@ -693,7 +675,7 @@ __device__ double newton_vjp(AdjSharedT<NF> &A, const DevGeom &g, const DevNum &
     for (int f = tid; f <= nl; f += LPM) A.lam[f] = f < nl ? S.nw.sys[src][3][f] * ca_drcp(S.nw.sys[src][1][f]) : 0.0;
     sync();
     // faces 1 .. nl - 1: cotangents of the Schur row entries and rhs, then
-    // of the 11 values the face row reads (dual numbers); W[0..3] centre
+    // of the 11 values the face row reads (face_vjp); W[0..3] centre
     // below, W[4..7] centre above, W[8..10] u_3 of faces f-1, f, f+1;
     // W[11..14] the row's r0, e0, r1, e1 for the centre-row pass
     for (int f = tid; f <= nl; f += LPM) {
@@ -705,7 +687,6 @@ __device__ double newton_vjp(AdjSharedT<NF> &A, const DevGeom &g, const DevNum &
             float prim[NP] = {Ut[0][f - 1], Ut[1][f - 1], Ut[2][f - 1], Ut[3][f - 1],
                               Ut[0][f],     Ut[1][f],     Ut[2][f],     Ut[3][f],
                               f - 1 >= 1 ? Ut[4][f - 1] : 0.0f, Ut[4][f], f + 1 <= nl - 1 ? Ut[4][f + 1] : 0.0f};
-            const FaceOut o = face_dual(g, n, f, dtg, prim);
             const double lam = A.lam[f];
             const double xm = S.nw.x[f - 1], x0 = S.nw.x[f], xp = S.nw.x[f + 1];
             const bool hasU = f + 1 <= nl - 1;
@@ -722,21 +703,17 @@ __device__ double newton_vjp(AdjSharedT<NF> &A, const DevGeom &g, const DevNum &
                                   Lb * pe0 + Db * pe1 + qb * pRe,
                                   Db * e0 + Ub * e1 + qb * Re,
                                   qb};
-            const D11 *outs[8] = {&o.lo, &o.d, &o.up, &o.r0, &o.r1, &o.e0, &o.e1, &o.Ru3};
+            double pbd[NP];
+            const FaceVals o = face_vjp(g, n, f, dtg, prim, ob, pbd);
 #pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) acc += ob[j] * outs[j]->d[i];
-                pb[i] = (float)acc;
-            }
+            for (int i = 0; i < NP; ++i) pb[i] = (float)pbd[i];
             if (f - 1 < 1) pb[8] = 0.0f;
             if (!hasU) pb[10] = 0.0f;
             if (n.rayleigh) awbar += g.zeta_f[f] * (-(double)dtg) * (Db + qb * (double)prim[9]);
-            r0v = (float)o.r0.v;
-            e0v = (float)o.e0.v;
-            r1v = (float)o.r1.v;
-            e1v = (float)o.e1.v;
+            r0v = (float)o.r0;
+            e0v = (float)o.e0;
+            r1v = (float)o.r1;
+            e1v = (float)o.e1;
         }
 #pragma unroll
         for (int i = 0; i < NP; ++i) W[i][f] = pb[i];

@@ -533,6 +533,18 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     HIPCHK(h, hipMemset(h->L, 0, 4 * (size_t)g.nC * sizeof(float)));
     ars343(h);
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
+    // level chunks sized so that small grids still fill the chip: about
+    // 1024 wavefronts for the per-node sweeps, 1024 workgroups for the DSS
+    {
+        const int64_t layer_waves = (nc + 63) / 64;
+        const int64_t want = (1024 + layer_waves - 1) / layer_waves;
+        h->explicit_chunk = (int)std::min<int64_t>(16, std::max<int64_t>(4, (nl + want - 1) / want));
+        h->adj_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
+        if (g.ncl > 0) {
+            const int64_t wantd = (1024 + g.ncl - 1) / g.ncl;
+            h->dss_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(2, (nl + wantd - 1) / wantd));
+        }
+    }
     return CA_OK;
 }
 

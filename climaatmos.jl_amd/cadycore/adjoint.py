@@ -1,9 +1,15 @@
-"""Differentiable column-ensemble solve: ``Y_N = step^nsteps(Y_0; D0, alpha_w)``
-as a ``torch.autograd.Function`` whose backward is the native reverse mode
-(``ca_ensemble_adjoint``: checkpointing every ``every`` steps, transposed
-Schur solve, reverse ARS343 stage sweep).  It is the Python counterpart of a
-ChainRulesCore ``rrule`` around the same C entry point; the reference itself
-has no adjoint (it calibrates without gradients, ``calibration/``).
+"""Differentiable solves as ``torch.autograd.Function``s whose backward is the
+native reverse mode:
+
+* ``EnsembleSolve``: column ensembles, ``Y_N = step^nsteps(Y_0; D0, alpha_w)``
+  (``ca_ensemble_adjoint``: checkpointing every ``every`` steps, transposed
+  Schur solve, reverse ARS343 stage sweep);
+* ``StepSolve``: the sphere, ``Y_N = step^nsteps(Y_0; D0)`` (forward
+  ``ca_step_record`` keeps checkpoints, backward ``ca_step_pullback``).
+
+They are the Python counterparts of ChainRulesCore ``rrule``s around the same
+C entry points; the reference itself has no adjoint (it calibrates without
+gradients, ``calibration/``).
 
 ``column_temperature`` and ``column_loss`` restate the calibration loss of the
 workload: the mean over columns of (integral of T dz - target)^2, with T from
@@ -54,6 +60,42 @@ class EnsembleSolve:
 
             cls._fn = _Solve
         return cls._fn.apply(Y0, D0, alpha_w, ens, dt, nsteps, every)
+
+
+class StepSolve:
+    """``Y_N = StepSolve.apply(Y0, D0, dycore, dt, nsteps, every)`` on a
+    sphere ``cadycore.lib.DryDycore`` created with vertical diffusion; D0 is
+    the per-column diffusivity (any float dtype/device, copied to the
+    dycore).  Differentiable in Y0 and D0."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, Y0, D0, dycore, dt, nsteps, every=8):
+        if cls._fn is None:
+            torch = _torch()
+
+            class _Solve(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, Y0, D0, dycore, dt, nsteps, every):
+                    dycore.set_column_params(D0_col=D0.detach().cpu().double().numpy())
+                    Y = Y0.detach().clone().contiguous()
+                    dycore.step_record(Y, dt, nsteps, every)
+                    ctx.dycore = dycore
+                    ctx.pdtype = (D0.dtype, D0.device)
+                    return Y
+
+                @staticmethod
+                def backward(ctx, Ybar):
+                    t = _torch()
+                    d = ctx.dycore
+                    D0b = t.zeros(d.grid.ncol, dtype=t.float32, device=Ybar.device)
+                    Yb0 = d.step_pullback(Ybar.contiguous().float(), D0b)
+                    dd, ddev = ctx.pdtype
+                    return Yb0, D0b.to(dtype=dd, device=ddev), None, None, None, None
+
+            cls._fn = _Solve
+        return cls._fn.apply(Y0, D0, dycore, dt, nsteps, every)
 
 
 def column_temperature(Y, members, params: Params):

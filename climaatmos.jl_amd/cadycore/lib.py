@@ -75,9 +75,10 @@ EXPORTED = (
     "ca_ensemble_create", "ca_ensemble_destroy", "ca_ensemble_state_size", "ca_ensemble_member_offset",
     "ca_ensemble_last_error", "ca_ensemble_step", "ca_ensemble_step_graph", "ca_ensemble_set_tuning",
     "ca_ensemble_get_tuning", "ca_ensemble_autotune", "ca_ensemble_set_params", "ca_ensemble_adjoint",
-    "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint",
+    "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint", "ca_step_record", "ca_step_pullback",
+    "ca_set_column_params",
 )
-CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND = 0, 1, 2, 3
+CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND, CA_TUNE_ADJ_CHUNK = 0, 1, 2, 3, 4
 CA_ENS_LANES, CA_ENS_ORDER = 0, 1
 _NOT_STATUS = ("ca_state_size", "ca_last_error", "ca_profile_count", "ca_ensemble_state_size",
                "ca_ensemble_member_offset", "ca_ensemble_last_error")
@@ -129,6 +130,9 @@ def load_library(path: str = LIB_PATH):
     lib.ca_dss_vjp.argtypes = [vp, vp, vp]
     lib.ca_remaining_tendency_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.ca_step_adjoint.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, vp]
+    lib.ca_step_record.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, vp]
+    lib.ca_step_pullback.argtypes = [vp, vp, vp, vp, vp]
+    lib.ca_set_column_params.argtypes = [vp, dpp, dpp]
     for nm in EXPORTED:
         if nm not in _NOT_STATUS:
             getattr(lib, nm).restype = C.c_int
@@ -290,6 +294,30 @@ class DryDycore:
         self._check(self.lib.ca_step_adjoint(self.h, self._ptr(Y0), self._ptr(YbN), float(dt), int(nsteps),
                                              int(every), self._ptr(Yb0), d0, self._stream()), "step adjoint")
         return Yb0
+
+    def step_record(self, Y, dt, nsteps, every=8):
+        """``step`` that keeps its checkpoints for ``step_pullback``."""
+        self._check(self.lib.ca_step_record(self.h, self._ptr(Y), float(dt), int(nsteps), int(every), self._stream()),
+                    "step record")
+
+    def step_pullback(self, YbN, D0b=None):
+        """Pullback of the last ``step_record``: returns the cotangent of its
+        input (and fills D0b, [ncol] float32)."""
+        Yb0 = self.zeros()
+        d0 = C.c_void_p(D0b.data_ptr()) if D0b is not None else None
+        self._check(self.lib.ca_step_pullback(self.h, self._ptr(YbN), self._ptr(Yb0), d0, self._stream()),
+                    "step pullback")
+        return Yb0
+
+    def set_column_params(self, alpha_w_col=None, D0_col=None):
+        def arr(x):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+            assert a.size == self.grid.ncol
+            self._keep[f"p{id(a)}"] = a
+            return _dp(a)
+        self._check(self.lib.ca_set_column_params(self.h, arr(alpha_w_col), arr(D0_col)), "set_column_params")
 
     def check_finite(self, Y):
         flag = C.c_int32(0)

@@ -205,6 +205,7 @@ __device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, 
 // --------------------------------------------------------------- LDS layout
 template <int NF>
 struct AdjSharedT {
+    static constexpr int NFv = NF;
     ens::SharedT<NF> S;       // forward recompute
     float Ut[3][5][NF];       // U~_i of the step being reversed
     float Us[3][5][NF];       // U_i
@@ -216,6 +217,18 @@ struct AdjSharedT {
     float W[24][NF];          // scratch of the local reverse passes
     double lam[NF];           // transposed Schur solution
     double red[2][128];       // block reduction of the parameter cotangents (LPM <= 128)
+};
+
+// The part of AdjSharedT one implicit stage's reverse uses (newton_vjp alone)
+template <int NF>
+struct NewtonAdjShared {
+    static constexpr int NFv = NF;
+    ens::StageShared<NF> S;
+    float Ut[1][5][NF];
+    float Ub[5][NF];
+    float Utb[5][NF];
+    float W[24][NF];
+    double lam[NF];
 };
 
 __device__ __forceinline__ void sync() { __syncthreads(); }
@@ -521,9 +534,10 @@ __device__ __forceinline__ void h_grad(const DevGeom &g, const DevNum &n, int k,
 // A.Ub holds the cotangent of U_i; on exit A.Utb holds the cotangent of
 // U~_i.  Returns this thread's alpha_w cotangent.  Runs ens::newton_stage on
 // U~_i first to rebuild the stage's internals.
-template <int LPM, int NF>
-__device__ double newton_vjp(AdjSharedT<NF> &A, const DevGeom &g, const DevNum &n, float dtg, const float (*Ut)[NF],
-                            float (*fi)[NF], int tid) {
+template <int LPM, class AS>
+__device__ double newton_vjp(AS &A, const DevGeom &g, const DevNum &n, float dtg, const float (*Ut)[AS::NFv],
+                            float (*fi)[AS::NFv], int tid) {
+    constexpr int NF = AS::NFv;
     auto &S = A.S;
     const int nl = g.nlev;
     float (*W)[NF] = A.W;

@@ -75,6 +75,10 @@ struct ca_handle {
     int64_t aw_rec_steps = 0;
     float *aw_ck = nullptr;       // checkpoints
     int64_t aw_ck_n = 0;
+    // the trajectory ca_step_record last stored (for ca_step_pullback)
+    bool rec_valid = false;
+    double rec_dt = 0.0;
+    int rec_nsteps = 0, rec_every = 0;
 };
 
 namespace {
@@ -220,10 +224,12 @@ ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
 // With dss_out, the tendency is written un-DSSed to the scratch buffer and
 // gathered (DSSed) into Yt; otherwise it goes to Yt as the reference's
 // remaining_tendency! returns it.
-ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st, bool dss_out = false) {
+ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st, bool dss_out = false,
+                            float *Lout = nullptr) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
     const bool dss = n.sphere;
+    float *Lw = Lout ? Lout : h->L;  // the DSSed Laplacians
     if (n.sphere && n.hyper) {
         {
             Launch l(h, st, KID_LAPLACIANS);
@@ -231,7 +237,7 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
         }
         {
             Launch l(h, st, KID_DSS_LAPLACIAN);
-            launch_dss<4, 0, false>(h, h->raw, h->L, 0, st);
+            launch_dss<4, 0, false>(h, h->raw, Lw, 0, st);
         }
     }
     float *out = (dss_out && dss) ? h->raw : Yt;
@@ -242,10 +248,10 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
         const dim3 grid(cdiv(g.ncol, 256), nchunks), blk(256);
         const bool hy = n.hyper != 0, vd = n.vdiff != 0;
         if (n.sphere) {
-            if (hy && !vd) k_explicit<true, true, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
-            else if (!hy && !vd) k_explicit<true, false, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
-            else if (hy && vd) k_explicit<true, true, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
-            else k_explicit<true, false, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
+            if (hy && !vd) k_explicit<true, true, false><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
+            else if (!hy && !vd) k_explicit<true, false, false><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
+            else if (hy && vd) k_explicit<true, true, true><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
+            else k_explicit<true, false, true><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
         } else {
             if (vd) k_explicit<false, false, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
             else k_explicit<false, false, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
@@ -258,24 +264,28 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
 }
 
 template <int TC>
-void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, hipStream_t st) {
+void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st) {
     const DevGeom &g = h->g;
     const dim3 grid(cdiv(g.ncol, TC)), blk(256);
     if (src.nfull == 2 && src.nfi == 0)
-        k_newton_pcr<TC, 2, 0><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+        k_newton_pcr<TC, 2, 0><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
     else if (src.nfull == 3 && src.nfi == 1)
-        k_newton_pcr<TC, 3, 1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+        k_newton_pcr<TC, 3, 1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
     else if (src.nfull == 4 && src.nfi == 2)
-        k_newton_pcr<TC, 4, 2><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+        k_newton_pcr<TC, 4, 2><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
     else
-        k_newton_pcr<TC, -1, -1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i]);
+        k_newton_pcr<TC, -1, -1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
 }
 
-void launch_newton_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, hipStream_t st) {
+// the implicit stage i from src into U (h->U unless the caller records);
+// Ut (optional) receives the assembled U~
+void launch_newton_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, hipStream_t st, float *U = nullptr,
+                       float *Ut = nullptr) {
+    if (!U) U = h->U;
     if (h->newton_variant == 2)
-        launch_pcr_tc<32>(h, src, dtg, i, st);
+        launch_pcr_tc<32>(h, src, dtg, i, U, Ut, st);
     else
-        launch_pcr_tc<16>(h, src, dtg, i, st);
+        launch_pcr_tc<16>(h, src, dtg, i, U, Ut, st);
 }
 
 ca_status ensure_bands(ca_handle *h) {
@@ -562,10 +572,9 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
     const DevGeom &g = h->g;
     ca_status s;
     // stage 1: explicit tendency at u (DSSed)
+    // recording: U_0 copied, U_i and the Laplacians written into their record slots
     if (rec) HIPCHK(h, hipMemcpyAsync(rec_U(h, rec, 0), Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
-    if ((s = explicit_tendency(h, h->FE[0], Y, st, true))) return s;
-    if (rec && h->n.hyper)
-        HIPCHK(h, hipMemcpyAsync(rec_L(h, rec, 0), h->L, 4 * (size_t)g.nC * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if ((s = explicit_tendency(h, h->FE[0], Y, st, true, rec ? rec_L(h, rec, 0) : nullptr))) return s;
     for (int i = 1; i < 4; ++i) {
         StageSrc src{};
         src.full[0] = Y;
@@ -585,24 +594,21 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
                 ++src.nfi;
             }
         float dtg = (float)(dt * h->aI[i][i]);
-        if (rec) {
+        const bool pcr = h->newton_variant == 1 || h->newton_variant == 2;
+        if (rec && !pcr) {  // the PCR stage records U~ itself
             sadj::k_stage_state<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, src, rec_Ut(h, rec, i));
             if ((s = check_launch(h, "k_stage_state"))) return s;
         }
+        float *Ui = rec ? rec_U(h, rec, i) : h->U;
         {
             Launch l(h, st, KID_NEWTON_STAGE);
-            if (h->newton_variant == 1 || h->newton_variant == 2)
-                launch_newton_pcr(h, src, dtg, i, st);
+            if (pcr)
+                launch_newton_pcr(h, src, dtg, i, st, Ui, rec ? rec_Ut(h, rec, i) : nullptr);
             else
-                k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, h->U, h->FI[i], h->scr);
+                k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, Ui, h->FI[i], h->scr);
         }
         if ((s = check_launch(h, "k_newton_stage"))) return s;
-        if (rec)
-            HIPCHK(h, hipMemcpyAsync(rec_U(h, rec, i), h->U, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
-        if ((s = explicit_tendency(h, h->FE[i], h->U, st, true))) return s;
-        if (rec && h->n.hyper)
-            HIPCHK(h, hipMemcpyAsync(rec_L(h, rec, i), h->L, 4 * (size_t)g.nC * sizeof(float), hipMemcpyDeviceToDevice,
-                                     st));
+        if ((s = explicit_tendency(h, h->FE[i], Ui, st, true, rec ? rec_L(h, rec, i) : nullptr))) return s;
     }
     // u_{n+1} = u + dt sum b (F_E + F_I), then dss! and the filter
     FinalSrc fs{};
@@ -866,6 +872,13 @@ ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value) {
         h->dss_kind = value;
         return CA_OK;
     }
+    if (knob == CA_TUNE_ADJ_CHUNK) {
+        if (value < 1 || value > h->g.nlev) return fail(h, CA_ERR_ARG, "adjoint chunk must be in [1, nlev]");
+        if (h->aw_d0 && value < h->adj_chunk)
+            return fail(h, CA_ERR_ARG, "adjoint chunk cannot shrink once the reverse-mode workspace exists");
+        h->adj_chunk = value;
+        return CA_OK;
+    }
     return fail(h, CA_ERR_ARG, "unknown tuning knob");
 }
 
@@ -875,6 +888,7 @@ ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value) {
     else if (knob == CA_TUNE_EXPLICIT_CHUNK) *value = h->explicit_chunk;
     else if (knob == CA_TUNE_DSS_CHUNK) *value = h->dss_chunk;
     else if (knob == CA_TUNE_DSS_KIND) *value = h->dss_kind;
+    else if (knob == CA_TUNE_ADJ_CHUNK) *value = h->adj_chunk;
     else return CA_ERR_ARG;
     return CA_OK;
 }

@@ -28,10 +28,25 @@ struct EnsTab {
 
 namespace ens {
 
+// Newton-stage intermediates of one column (LDS)
+template <int NF>
+struct NwT {
+    float c[11][NF];       // rho, K, T, p, Pi, ct1, ct2, h, J, iJ, rJ
+    double cd[3][NF];      // Td, pd, Pid
+    float row[6][NF];      // a0, a1, e0, e1, Rr, Re
+    double sys[2][4][NF];  // PCR ping-pong: L, D, U, r
+    float x[NF];           // du_3
+    float dF[NF];          // upwind - central flux difference
+    float hr[NF];          // h relative to the lowest cell, new state
+    float rJ[NF];          // rho J, new state
+    float ve[NF];          // rho e_tot after the Newton update, before the correction
+};
+
 // LDS of one member.  The scratch union holds the explicit-tendency or the
 // Newton-stage intermediates.
 template <int NF>
 struct SharedT {
+    static constexpr int NFv = NF;
     float u[5][NF];      // state at the start of the step (4 centre planes, u_3)
     float fe[4][5][NF];  // explicit tendencies of the four stages
     float fi[3][3][NF];  // implicit tendencies of stages 1..3: rho, rho e_tot, u_3
@@ -41,18 +56,17 @@ struct SharedT {
             float c[10][NF];  // rhoJ, Kh, Kv, u1, u2, rho, sd, irhoJ, d1, d2
             float f[5][NF];   // G1, G2, Fs, Fu0, Fu1
         } ex;
-        struct {
-            float c[11][NF];    // rho, K, T, p, Pi, ct1, ct2, h, J, iJ, rJ
-            double cd[3][NF];   // Td, pd, Pid
-            float row[6][NF];   // a0, a1, e0, e1, Rr, Re
-            double sys[2][4][NF];  // PCR ping-pong: L, D, U, r
-            float x[NF];        // du_3
-            float dF[NF];       // upwind - central flux difference
-            float hr[NF];       // h relative to the lowest cell, new state
-            float rJ[NF];       // rho J, new state
-            float ve[NF];       // rho e_tot after the Newton update, before the correction
-        } nw;
+        NwT<NF> nw;
     };
+};
+
+// The part of SharedT one implicit stage uses (newton_stage alone)
+template <int NF>
+struct StageShared {
+    static constexpr int NFv = NF;
+    float fi[1][3][NF];
+    float U[5][NF];
+    NwT<NF> nw;
 };
 
 enum { EC_RHOJ, EC_KH, EC_KV, EC_U1, EC_U2, EC_RHO, EC_SD, EC_IRHOJ, EC_D1, EC_D2 };
@@ -61,8 +75,8 @@ enum { NC_RHO, NC_K, NC_T, NC_P, NC_PI, NC_CT1, NC_CT2, NC_H, NC_J, NC_IJ, NC_RJ
 __device__ __forceinline__ void sync() { __syncthreads(); }
 
 // CV of centre k from the Newton scratch
-template <int NF>
-__device__ __forceinline__ CV cv_from(const SharedT<NF> &S, int k) {
+template <class SH>
+__device__ __forceinline__ CV cv_from(const SH &S, int k) {
     CV c;
     c.rho = S.nw.c[NC_RHO][k];
     c.K = S.nw.c[NC_K][k];
@@ -189,8 +203,8 @@ __device__ void explicit_tendency(SharedT<NF> &S, const DevGeom &g, const DevNum
 // F_I = (U - U~) / dtg for rho, rho e_tot and u_3.  Same algorithm as
 // k_newton_pcr (one Newton iteration from U~, manual Jacobian, Schur
 // complement on u_3, post-Newton upwind correction).
-template <int LPM, int NF>
-__device__ void newton_stage(SharedT<NF> &S, const DevGeom &g, const DevNum &n, float dtg, float (*fi)[NF],
+template <int LPM, class SH>
+__device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg, float (*fi)[SH::NFv],
                              int tid) {
     const int nl = g.nlev;
     const float idtg = 1.0f / dtg;

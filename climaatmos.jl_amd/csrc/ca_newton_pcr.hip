@@ -61,7 +61,7 @@ __device__ __forceinline__ void assemble(const StageSrc &src, int64_t nC, int64_
 
 template <int TC, int NF, int NI>
 __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U,
-                                                    float *FI) {
+                                                    float *FI, float *Ut) {
     constexpr int R = TC + 1;  // padded LDS row (conflict-free column reads)
     __shared__ float tU[5][64][R];
     __shared__ float tF[3][64][R];
@@ -79,6 +79,14 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         assemble<NF, NI>(src, nC, (int64_t)r * nc + col, col < nc, r < nl, r >= 1 && r <= nl - 1, v);
 #pragma unroll
         for (int comp = 0; comp < 5; ++comp) tU[comp][r][cc] = v[comp];
+        // U~ for the reverse pass (when recording), in the state layout
+        if (Ut && col < nc) {
+            const int64_t q = (int64_t)r * nc + col;
+            if (r < nl)
+#pragma unroll
+                for (int comp = 0; comp < 4; ++comp) Ut[comp * nC + q] = v[comp];
+            if (r <= nl) Ut[4 * nC + q] = v[4];
+        }
     }
     __syncthreads();
 
@@ -238,7 +246,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
 }
 
 #define CA_INST_PCR(TC, NF, NI) \
-    template __global__ void k_newton_pcr<TC, NF, NI>(DevGeom, DevNum, StageSrc, float, float *, float *);
+    template __global__ void k_newton_pcr<TC, NF, NI>(DevGeom, DevNum, StageSrc, float, float *, float *, float *);
 CA_INST_PCR(16, 2, 0)
 CA_INST_PCR(16, 3, 1)
 CA_INST_PCR(16, 4, 2)

@@ -73,8 +73,8 @@ __device__ __forceinline__ float dtref(const DevNum &n, float Pi) {
 
 // ------------------------------------------------ Laplacians (k_laplacians)
 // Given Lb, the cotangent of k_laplacians' output at (node, level k), adds
-// the centre-state cotangents of the same (node, level) to Yb and writes the
-// u_3 cotangents of faces k and k + 1 to fa[k], fb[k] (zero at the boundary
+// the centre-state cotangents of the same (node, level) to Yb and the u_3
+// cotangents of faces k and k + 1 to fa[k], fb[k] (nothing at the boundary
 // faces, which the forward reads as 0).
 __global__ void __launch_bounds__(256) k_laplacians_vjp(DevGeom g, DevNum n, const float *Y, const float *Lb,
                                                         float *Yb, float *fa, float *fb) {
@@ -153,8 +153,8 @@ __global__ void __launch_bounds__(256) k_laplacians_vjp(DevGeom g, DevNum n, con
     Yb[nC + idx] += u1b;
     Yb[2 * nC + idx] += u2b;
     Yb[3 * nC + idx] += rhoeb;
-    fa[idx] = k >= 1 ? u3ab : 0.0f;
-    fb[idx] = k + 1 <= nl - 1 ? u3bb : 0.0f;
+    if (k >= 1) fa[idx] += u3ab;
+    if (k + 1 <= nl - 1) fb[idx] += u3bb;
 }
 
 // ------------------------------------------------ explicit tendency (k_explicit)
@@ -438,16 +438,19 @@ struct FaceF {
 // the state and the DSSed Laplacians it was evaluated at.  Writes the state
 // cotangent Yb (centre planes of the owned centres, u_3 of the owned faces),
 // the cotangent Lb of the Laplacians (owned centres), and adds the D0
-// cotangent of the owned faces to d0acc[chunk * ncol + col].
+// cotangent of the owned faces to d0acc[chunk * ncol + col].  The u_3
+// cotangent is split: Yb's face plane gets each face's own terms, and the
+// local reverse of centre k writes its u_3 terms for faces k and k + 1 to
+// fa[k], fb[k] (zero on the boundary faces), which the caller adds.
 // Work item = (node, chunk of levels) as k_explicit; chunk c owns centres
 // and faces [k0, k1) (and face nlev when k1 = nlev).  The sweep goes up from
-// centre k0 - 2: face f's reverse gives the centres f - 1 and f their face
-// cotangents, after which centre f - 1 is complete and its local reverse
-// runs.
+// centre k0 - 1: face f's reverse gives the centres f - 1 and f their face
+// cotangents, after which centre f - 1 is complete and, if owned, its local
+// reverse runs.
 template <bool HYPER, bool VDIFF>
 __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const float *Y, const float *L,
-                                                      const float *Ytb, float *Yb, float *Lb, double *d0acc,
-                                                      int chunk) {
+                                                      const float *Ytb, float *Yb, float *Lb, float *fa, float *fb,
+                                                      double *d0acc, int chunk) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.ncol) return;
     const Nod q = ca_node(threadIdx.x & 63);
@@ -467,14 +470,13 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
     }
     const int k0 = chunk > 0 ? (int)blockIdx.y * chunk : 0;
     const int k1 = chunk > 0 ? min(k0 + chunk, nl) : nl;
-    const int kbeg = max(k0 - 2, 0);
+    const int kbeg = max(k0 - 1, 0);
     Loc prev, cur;
     LocBar pb, cb;
     lb_zero(pb);
     lb_zero(cb);
     float pY[4] = {0, 0, 0, 0}, cY[4] = {0, 0, 0, 0};  // tendency cotangents of prev / cur centres
     FaceF fprev{0, 0, 0, 0, 0, 0};                      // face below prev
-    float u3carry = 0.0f;  // cotangent of u_3 at face k - 1 so far (face reverse + centre below)
     double d0 = 0.0;
     for (int k = kbeg; k <= k1; ++k) {
         // ---- centre k: forward, tendency cotangent
@@ -592,8 +594,9 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
                 }
             }
         }
-        // ---- centre k - 1 complete: completion terms that need both of its faces, then its local reverse
-        if (k - 1 >= max(k0 - 1, 0) && k > kbeg) {
+        if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) Yb[4 * nC + (int64_t)k * nc + col] = u3face;
+        // ---- owned centre k - 1 complete: completion terms that need both of its faces, then its local reverse
+        if (k - 1 >= k0 && k > kbeg) {
             const int kk = k - 1;
             pb.irhoJ += -0.5f * (fprev.G1 + fk.G1) * pY[1] - 0.5f * (fprev.G2 + fk.G2) * pY[2];
             if (VDIFF) {
@@ -607,23 +610,18 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
             }
             float ubc[4], Lbc[4], u3ab, u3bb;
             loc_reverse<HYPER, VDIFF>(prev, pb, pY, g, n, q, col, kk, ubc, u3ab, u3bb, Lbc);
-            if (kk >= k0 && kk < k1) {
-                const int64_t idx = (int64_t)kk * nc + col;
-                Yb[idx] = ubc[0];
-                Yb[nC + idx] = ubc[1];
-                Yb[2 * nC + idx] = ubc[2];
-                Yb[3 * nC + idx] = ubc[3];
-                Lb[idx] = Lbc[0];
-                Lb[nC + idx] = Lbc[1];
-                Lb[2 * nC + idx] = Lbc[2];
-                Lb[3 * nC + idx] = Lbc[3];
-                // face kk complete: its own reverse and centres kk - 1, kk
-                Yb[4 * nC + idx] = (kk >= 1) ? u3carry + u3ab : 0.0f;
-            }
-            u3face += u3bb;
+            const int64_t idx = (int64_t)kk * nc + col;
+            Yb[idx] = ubc[0];
+            Yb[nC + idx] = ubc[1];
+            Yb[2 * nC + idx] = ubc[2];
+            Yb[3 * nC + idx] = ubc[3];
+            Lb[idx] = Lbc[0];
+            Lb[nC + idx] = Lbc[1];
+            Lb[2 * nC + idx] = Lbc[2];
+            Lb[3 * nC + idx] = Lbc[3];
+            fa[idx] = kk >= 1 ? u3ab : 0.0f;
+            fb[idx] = kk + 1 <= nl - 1 ? u3bb : 0.0f;
         }
-        if (k == nl && k1 == nl && k >= k0) Yb[4 * nC + (int64_t)nl * nc + col] = 0.0f;
-        u3carry = u3face;
         fprev = fk;
         prev = cur;
         pb = cb;
@@ -636,13 +634,13 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
 }
 
 template __global__ void k_explicit_vjp<true, true>(DevGeom, DevNum, const float *, const float *, const float *,
-                                                    float *, float *, double *, int);
+                                                    float *, float *, float *, float *, double *, int);
 template __global__ void k_explicit_vjp<true, false>(DevGeom, DevNum, const float *, const float *, const float *,
-                                                     float *, float *, double *, int);
+                                                     float *, float *, float *, float *, double *, int);
 template __global__ void k_explicit_vjp<false, true>(DevGeom, DevNum, const float *, const float *, const float *,
-                                                     float *, float *, double *, int);
+                                                     float *, float *, float *, float *, double *, int);
 template __global__ void k_explicit_vjp<false, false>(DevGeom, DevNum, const float *, const float *, const float *,
-                                                      float *, float *, double *, int);
+                                                      float *, float *, float *, float *, double *, int);
 
 // ------------------------------------------------ implicit stage, per column
 // Reverse of one Newton stage (k_newton_pcr) on column blockIdx.x: given
@@ -651,41 +649,55 @@ template __global__ void k_explicit_vjp<false, false>(DevGeom, DevNum, const flo
 // sphere's per-column planes offset to the column) and reversed by the
 // column ensemble's adj::newton_vjp, which re-runs the stage first.
 template <int LPM, int NF>
-__global__ void __launch_bounds__(LPM) k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc,
+__global__ void __launch_bounds__(LPM) __attribute__((amdgpu_waves_per_eu(4))) k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc,
                                                     const float *__restrict__ Ut, const float *__restrict__ Ub,
-                                                    float dtg, float *__restrict__ Utb) {
+                                                    float dtg, float *__restrict__ Utb, const float *__restrict__ fib,
+                                                    float cfi, const float *__restrict__ fa,
+                                                    const float *__restrict__ fb) {
     using namespace adj;
-    __shared__ AdjSharedT<NF> A;
+    __shared__ NewtonAdjShared<NF> A;
     const int col = blockIdx.x;
     const int tid = threadIdx.x;
     const DevGeom &g = cg[col];
     const int nl = g.nlev;
+    // the cotangent of U: Ub, + F_I's direct term -cfi fib (rho, rho e_tot,
+    // u_3) and the Laplacian reverse's face terms fa, fb when given
     for (int k = tid; k < nl; k += LPM)
         for (int p = 0; p < 4; ++p) {
             const int64_t i = p * nC + (int64_t)k * nc + col;
             const float v = Ut[i];
             A.S.U[p][k] = v;
             A.Ut[0][p][k] = v;
-            A.Ub[p][k] = Ub[i];
+            A.Ub[p][k] = (p == 0 || p == 3) ? fmaf(-cfi, fib[i], Ub[i]) : Ub[i];
         }
     for (int f = tid; f <= nl; f += LPM) {
         const bool in = f >= 1 && f <= nl - 1;
-        const int64_t i = 4 * nC + (int64_t)f * nc + col;
+        const int64_t q = (int64_t)f * nc + col, i = 4 * nC + q;
         const float v = in ? Ut[i] : 0.0f;
         A.S.U[4][f] = v;
         A.Ut[0][4][f] = v;
-        A.Ub[4][f] = in ? Ub[i] : 0.0f;
+        float b = in ? fmaf(-cfi, fib[i], Ub[i]) : 0.0f;
+        if (in && fa) b += fa[q] + fb[q - nc];
+        A.Ub[4][f] = b;
     }
     __syncthreads();
     (void)newton_vjp<LPM>(A, g, n, dtg, A.Ut[0], A.S.fi[0], tid);
+    // U~ also enters F_I = (U - U~) / dtg directly: Utb += cfi fib on rho, rho e_tot, u_3
     for (int k = tid; k < nl; k += LPM)
-        for (int p = 0; p < 4; ++p) Utb[p * nC + (int64_t)k * nc + col] = A.Utb[p][k];
-    for (int f = tid; f <= nl; f += LPM) Utb[4 * nC + (int64_t)f * nc + col] = A.Utb[4][f];
+        for (int p = 0; p < 4; ++p) {
+            const int64_t i = p * nC + (int64_t)k * nc + col;
+            Utb[i] = (p == 0 || p == 3) ? fmaf(cfi, fib[i], A.Utb[p][k]) : A.Utb[p][k];
+        }
+    for (int f = tid; f <= nl; f += LPM) {
+        const int64_t i = 4 * nC + (int64_t)f * nc + col;
+        Utb[i] = fmaf(cfi, fib[i], A.Utb[4][f]);
+    }
 }
 
 #define CA_NVJP_INST(LPM, NF)                                                                                        \
     template __global__ void k_newton_vjp<LPM, NF>(const DevGeom *, DevNum, int64_t, int64_t, const float *,        \
-                                                   const float *, float, float *);
+                                                   const float *, float, float *, const float *, float,          \
+                                                   const float *, const float *);
 CA_NVJP_INST(64, 33)
 CA_NVJP_INST(64, 65)
 CA_NVJP_INST(128, 129)
@@ -708,22 +720,39 @@ __global__ void __launch_bounds__(256) k_stage_state(DevGeom g, StageSrc src, fl
 }
 
 // out = (acc ? out : 0) + sum_t a_t x_t over the planes selected by the
-// bit mask (bit p: plane p, 0..3 centres, 4 faces).
+// bit mask (bit p: plane p, 0..3 centres, 4 faces); with fa, fb, the
+// interior faces of out also get fa[f] + fb[f - 1] (the Laplacian reverse's
+// face terms); with out2, out2 = sum_t a2_t x_t over planes2 as well.
 struct Lin {
     const float *x[6];
-    float a[6];
+    float a[6], a2[6];
     int nx, acc;
-    unsigned planes;
+    unsigned planes, planes2;
+    float *out2;
+    const float *fa, *fb;
 };
 __global__ void __launch_bounds__(256) k_lincomb(DevGeom g, float *out, Lin L) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t S = 4 * g.nC + g.nF;
     if (i >= S) return;
     const int p = i < 4 * g.nC ? (int)(i / g.nC) : 4;
-    if (!((L.planes >> p) & 1u)) return;
-    float v = L.acc ? out[i] : 0.0f;
-    for (int t = 0; t < L.nx; ++t) v = fmaf(L.a[t], L.x[t][i], v);
-    out[i] = v;
+    float x[6];
+    for (int t = 0; t < L.nx; ++t) x[t] = L.x[t][i];
+    if ((L.planes >> p) & 1u) {
+        float v = L.acc ? out[i] : 0.0f;
+        for (int t = 0; t < L.nx; ++t) v = fmaf(L.a[t], x[t], v);
+        if (p == 4 && L.fa) {
+            const int64_t q = i - 4 * g.nC;
+            const int64_t f = q / g.ncol;
+            if (f >= 1 && f <= g.nlev - 1) v += L.fa[q] + L.fb[q - g.ncol];
+        }
+        out[i] = v;
+    }
+    if (L.out2 && ((L.planes2 >> p) & 1u)) {
+        float v = 0.0f;
+        for (int t = 0; t < L.nx; ++t) v = fmaf(L.a2[t], x[t], v);
+        L.out2[i] = v;
+    }
 }
 
 // u_3 cotangent of the interior faces += the Laplacian reverse's face terms

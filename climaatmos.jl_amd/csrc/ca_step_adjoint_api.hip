@@ -1,12 +1,13 @@
 // ca_step_adjoint_api.hip -- host side of the sphere step's reverse mode:
-// ca_dss_vjp, ca_remaining_tendency_vjp and ca_step_adjoint.  Included after
+// ca_dss_vjp, ca_remaining_tendency_vjp, ca_step_record / ca_step_pullback,
+// ca_step_adjoint and ca_set_column_params.  Included after
 // ca_api.hip in the single translation unit (it uses the handle internals).
 #include "ca_kernels.h"
 
 namespace {
 
 // planes of the S-sized reverse-mode scratch h->aw_buf
-enum AdjBuf { AB_UB, AB_CB, AB_UTB1, AB_UTB2, AB_UTB3, AB_FEB, AB_FIB, AB_GB, AB_UBS, AB_Y, AB_COUNT };
+enum AdjBuf { AB_UB, AB_CB, AB_UTB1, AB_UTB2, AB_UTB3, AB_FEB, AB_FIB, AB_GB, AB_UBS, AB_Y, AB_YF, AB_COUNT };
 
 ca_status adj_ensure(ca_handle *h, int64_t rec_steps, int64_t nck) {
     const DevGeom &g = h->g;
@@ -64,41 +65,68 @@ ca_status dss_T_state(ca_handle *h, const float *src, float *dst, int filter, hi
     return check_launch(h, "k_dss(transposed state)");
 }
 
-ca_status lincomb(ca_handle *h, float *out, std::initializer_list<std::pair<const float *, double>> terms, bool acc,
-                  unsigned planes, hipStream_t st) {
+struct LinTerm {
+    const float *x;
+    double a, a2;
+};
+
+// out = (acc ? out : 0) + sum a x over `planes` (+ the Laplacian face terms
+// fa, fb on the interior faces); with out2: out2 = sum a2 x over planes2
+ca_status lincomb(ca_handle *h, float *out, std::initializer_list<LinTerm> terms, bool acc, unsigned planes,
+                  hipStream_t st, float *out2 = nullptr, unsigned planes2 = 0, const float *fa = nullptr,
+                  const float *fb = nullptr) {
     sadj::Lin L{};
     L.nx = 0;
     for (const auto &t : terms) {
-        if (t.second == 0.0) continue;
+        if (t.a == 0.0 && t.a2 == 0.0) continue;
         if (L.nx >= 6) return fail(h, CA_ERR_ARG, "lincomb: too many terms");
-        L.x[L.nx] = t.first;
-        L.a[L.nx] = (float)t.second;
+        L.x[L.nx] = t.x;
+        L.a[L.nx] = (float)t.a;
+        L.a2[L.nx] = (float)t.a2;
         ++L.nx;
     }
     L.acc = acc ? 1 : 0;
     L.planes = planes;
+    L.out2 = out2;
+    L.planes2 = planes2;
+    L.fa = fa;
+    L.fb = fb;
     sadj::k_lincomb<<<cdiv(h->S, 256), 256, 0, st>>>(h->g, out, L);
     return check_launch(h, "k_lincomb");
 }
 
 constexpr unsigned PL_ALL = 0x1F, PL_FI = (1u << 0) | (1u << 3) | (1u << 4);  // rho, rho e_tot, u_3
 
+ca_status fold_faces(ca_handle *h, float *Yb, const float *fa, const float *fb, hipStream_t st) {
+    sadj::k_fold_faces<<<dim3(cdiv(h->g.ncol, 256), h->g.nlev - 1), 256, 0, st>>>(h->g, Yb, fa, fb);
+    return check_launch(h, "k_fold_faces");
+}
+
 // Yb = (d T_E / d U)^T gb at U with the DSSed Laplacians L (hyperdiffusion);
 // D0 cotangents are added to h->aw_d0.  gb: cotangent of the un-DSSed
 // explicit tendency.
-ca_status explicit_vjp(ca_handle *h, const float *U, const float *L, const float *gb, float *Yb, hipStream_t st) {
+// With fold = false the u_3 face terms of the explicit and Laplacian
+// reverses are left in the handle's fa / fb planes (face_a, face_b) for the
+// caller to add.
+ca_status explicit_vjp(ca_handle *h, const float *U, const float *L, const float *gb, float *Yb, hipStream_t st,
+                       bool fold = true) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
     const int ch = h->adj_chunk;
     const dim3 grid(cdiv(g.ncol, 256), cdiv(g.nlev, ch)), blk(256);
     float *Lb = h->aw_L, *Lb2 = h->aw_L + 4 * g.nC, *fa = h->aw_L + 8 * g.nC, *fb = h->aw_L + 12 * g.nC;
     const bool hy = n.hyper != 0, vd = n.vdiff != 0;
-    if (hy && vd) sadj::k_explicit_vjp<true, true><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
-    else if (hy) sadj::k_explicit_vjp<true, false><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
-    else if (vd) sadj::k_explicit_vjp<false, true><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
-    else sadj::k_explicit_vjp<false, false><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, h->aw_d0, ch);
+    if (hy && vd)
+        sadj::k_explicit_vjp<true, true><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, fa, fb, h->aw_d0, ch);
+    else if (hy)
+        sadj::k_explicit_vjp<true, false><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, fa, fb, h->aw_d0, ch);
+    else if (vd)
+        sadj::k_explicit_vjp<false, true><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, fa, fb, h->aw_d0, ch);
+    else
+        sadj::k_explicit_vjp<false, false><<<grid, blk, 0, st>>>(g, n, U, L, gb, Yb, Lb, fa, fb, h->aw_d0, ch);
     ca_status s = check_launch(h, "k_explicit_vjp");
-    if (s || !hy) return s;
+    if (s) return s;
+    if (!hy) return fold && g.nlev >= 2 ? fold_faces(h, Yb, fa, fb, st) : CA_OK;
     {
         const DevGeom &gT = h->gT;
         const int dch = h->dss_chunk > 0 ? h->dss_chunk : gT.nlev;
@@ -110,23 +138,30 @@ ca_status explicit_vjp(ca_handle *h, const float *U, const float *L, const float
     if ((s = check_launch(h, "k_dss(transposed laplacians)"))) return s;
     sadj::k_laplacians_vjp<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, n, U, Lb2, Yb, fa, fb);
     if ((s = check_launch(h, "k_laplacians_vjp"))) return s;
-    if (g.nlev >= 2) {
-        sadj::k_fold_faces<<<dim3(cdiv(g.ncol, 256), g.nlev - 1), 256, 0, st>>>(g, Yb, fa, fb);
-        s = check_launch(h, "k_fold_faces");
-    }
-    return s;
+    return fold && g.nlev >= 2 ? fold_faces(h, Yb, fa, fb, st) : CA_OK;
 }
 
-// Utb = (d Newton_i / d U~)^T Ub at U~ = Ut, column by column
-ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Ub, float dtg, float *Utb, hipStream_t st) {
+// the face planes of the explicit and Laplacian reverses' u_3 terms
+const float *face_a(const ca_handle *h) { return h->aw_L + 8 * h->g.nC; }
+const float *face_b(const ca_handle *h) { return h->aw_L + 12 * h->g.nC; }
+
+// Utb = (d Newton_i / d U~)^T Ub at U~ = Ut, column by column, + cfi fib
+// on rho, rho e_tot and u_3
+// The cotangent of U is Ub - cfi fib (rho, rho e_tot, u_3) + the face terms
+// fa, fb (if given): F_I = (U - U~) / dtg with cfi = -1 / dtg.
+ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Ub, float dtg, float *Utb, const float *fib,
+                     float cfi, const float *fa, const float *fb, hipStream_t st) {
     const DevGeom &g = h->g;
     const dim3 grid(g.ncol);
     if (g.nlev <= 32)
-        sadj::k_newton_vjp<64, 33><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb);
+        sadj::k_newton_vjp<64, 33><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib, cfi, fa,
+                                                       fb);
     else if (g.nlev <= 64)
-        sadj::k_newton_vjp<64, 65><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb);
+        sadj::k_newton_vjp<64, 65><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib, cfi, fa,
+                                                       fb);
     else
-        sadj::k_newton_vjp<128, 129><<<grid, 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb);
+        sadj::k_newton_vjp<128, 129><<<grid, 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib, cfi,
+                                                          fa, fb);
     return check_launch(h, "k_newton_vjp");
 }
 
@@ -149,38 +184,38 @@ ca_status reverse_step(ca_handle *h, float *rec, double dt, hipStream_t st) {
     float *ub = abuf(h, AB_UB), *Cb = abuf(h, AB_CB), *feb = abuf(h, AB_FEB), *fib = abuf(h, AB_FIB);
     float *gb = abuf(h, AB_GB), *Ubs = abuf(h, AB_UBS);
     float *Utb[4] = {nullptr, abuf(h, AB_UTB1), abuf(h, AB_UTB2), abuf(h, AB_UTB3)};
+    const float *fa = face_a(h), *fb = face_b(h);
     // u' = DSS_f(C), C = u + dt sum_j b_j (F_E_j + F_I_j)
     if ((s = dss_T_state(h, ub, Cb, 1, st))) return s;
     for (int i = 3; i >= 1; --i) {
         const double dtg = dt * h->aI[i][i];
-        // cotangents of F_E_i and F_I_i: from C and from the later stages' U~
-        auto later = [&](double (*a)[4], int l) { return l > i ? dt * a[l][i] : 0.0; };
-        if ((s = lincomb(h, feb, {{Cb, dt * h->b[i]}, {Utb[1], later(h->aE, 1)}, {Utb[2], later(h->aE, 2)},
-                                  {Utb[3], later(h->aE, 3)}},
-                         false, PL_ALL, st)))
-            return s;
-        if ((s = lincomb(h, fib, {{Cb, dt * h->b[i]}, {Utb[1], later(h->aI, 1)}, {Utb[2], later(h->aI, 2)},
-                                  {Utb[3], later(h->aI, 3)}},
-                         false, PL_FI, st)))
+        // cotangents of F_E_i (feb) and F_I_i (fib): from C and the later stages' U~
+        auto ae = [&](int l) { return l > i ? dt * h->aE[l][i] : 0.0; };
+        auto ai = [&](int l) { return l > i ? dt * h->aI[l][i] : 0.0; };
+        if ((s = lincomb(h, feb,
+                         {{Cb, dt * h->b[i], dt * h->b[i]}, {Utb[1], ae(1), ai(1)}, {Utb[2], ae(2), ai(2)},
+                          {Utb[3], ae(3), ai(3)}},
+                         false, PL_ALL, st, fib, PL_FI)))
             return s;
         // F_E_i = DSS(T_E(U_i))
         if ((s = dss_T_state(h, feb, gb, 0, st))) return s;
-        if ((s = explicit_vjp(h, rec_U(h, rec, i), rec_L(h, rec, i), gb, Ubs, st))) return s;
-        // F_I_i = (U_i - U~_i) / dtg
-        if ((s = lincomb(h, Ubs, {{fib, 1.0 / dtg}}, true, PL_FI, st))) return s;
-        // U_i = Newton(U~_i)
-        if ((s = newton_vjp(h, rec_Ut(h, rec, i), Ubs, (float)dtg, Utb[i], st))) return s;
-        if ((s = lincomb(h, Utb[i], {{fib, -1.0 / dtg}}, true, PL_FI, st))) return s;
+        if ((s = explicit_vjp(h, rec_U(h, rec, i), rec_L(h, rec, i), gb, Ubs, st, false))) return s;
+        // U_i = Newton(U~_i); F_I_i = (U_i - U~_i) / dtg and the Laplacian
+        // face terms are added to the U_i cotangent as the kernel loads it
+        if ((s = newton_vjp(h, rec_Ut(h, rec, i), Ubs, (float)dtg, Utb[i], fib, (float)(-1.0 / dtg), fa, fb, st)))
+            return s;
     }
     // F_E_0 = DSS(T_E(u))
-    if ((s = lincomb(h, feb, {{Cb, dt * h->b[0]}, {Utb[1], dt * h->aE[1][0]}, {Utb[2], dt * h->aE[2][0]},
-                              {Utb[3], dt * h->aE[3][0]}},
+    if ((s = lincomb(h, feb,
+                     {{Cb, dt * h->b[0], 0.0}, {Utb[1], dt * h->aE[1][0], 0.0}, {Utb[2], dt * h->aE[2][0], 0.0},
+                      {Utb[3], dt * h->aE[3][0], 0.0}},
                      false, PL_ALL, st)))
         return s;
     if ((s = dss_T_state(h, feb, gb, 0, st))) return s;
-    if ((s = explicit_vjp(h, rec_U(h, rec, 0), rec_L(h, rec, 0), gb, Ubs, st))) return s;
+    if ((s = explicit_vjp(h, rec_U(h, rec, 0), rec_L(h, rec, 0), gb, Ubs, st, false))) return s;
     // u enters C and every U~_i with weight 1
-    return lincomb(h, ub, {{Cb, 1.0}, {Utb[1], 1.0}, {Utb[2], 1.0}, {Utb[3], 1.0}, {Ubs, 1.0}}, false, PL_ALL, st);
+    return lincomb(h, ub, {{Cb, 1.0, 0.0}, {Utb[1], 1.0, 0.0}, {Utb[2], 1.0, 0.0}, {Utb[3], 1.0, 0.0}, {Ubs, 1.0, 0.0}},
+                   false, PL_ALL, st, nullptr, 0, fa, fb);
 }
 
 }  // namespace
@@ -223,30 +258,45 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
     return CA_OK;
 }
 
-ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, double dt, int32_t nsteps, int32_t every,
-                          float *Yb0, float *D0b, void *stream) {
-    if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
-    if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
+ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int32_t every, void *stream) {
+    if (!h || !Y || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_record: sphere grids only (columns: ca_ensemble_adjoint)");
     hipStream_t st = (hipStream_t)stream;
     const int64_t S = h->S;
     const int nseg = (nsteps + every - 1) / every;
     ca_status s;
+    h->rec_valid = false;
     if ((s = adj_ensure(h, every, nseg))) return s;
-    float *Yw = abuf(h, AB_Y);
-    // forward: the step's initial dss! and filter, then checkpoints every `every` steps
-    HIPCHK(h, hipMemcpyAsync(h->U, Y0, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
-    if ((s = dss_gather_state(h, h->U, Yw, 1, st, KID_DSS_STATE))) return s;
-    HIPCHK(h, hipMemcpyAsync(h->aw_ck, Yw, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
-    for (int t = 0; t < (nseg - 1) * every; ++t) {
-        if ((s = one_step(h, Yw, dt, st, nullptr))) return s;
-        if ((t + 1) % every == 0)
-            HIPCHK(h, hipMemcpyAsync(h->aw_ck + (int64_t)((t + 1) / every) * S, Yw, (size_t)S * sizeof(float),
+    // the step's initial dss! and filter, then a checkpoint at every segment start
+    HIPCHK(h, hipMemcpyAsync(h->U, Y, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
+    for (int t = 0; t < nsteps; ++t) {
+        if (t % every == 0)
+            HIPCHK(h, hipMemcpyAsync(h->aw_ck + (int64_t)(t / every) * S, Y, (size_t)S * sizeof(float),
                                      hipMemcpyDeviceToDevice, st));
+        if ((s = one_step(h, Y, dt, st, nullptr))) return s;
     }
-    // reverse: per segment (last first), recompute with records, then reverse its steps
+    h->rec_valid = true;
+    h->rec_dt = dt;
+    h->rec_nsteps = nsteps;
+    h->rec_every = every;
+    return CA_OK;
+}
+
+ca_status ca_step_pullback(ca_handle *h, const float *YbN, float *Yb0, float *D0b, void *stream) {
+    if (!h || !YbN || !Yb0) return CA_ERR_ARG;
+    if (!h->rec_valid) return fail(h, CA_ERR_ARG, "ca_step_pullback: no trajectory recorded (ca_step_record)");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t S = h->S;
+    const int nsteps = h->rec_nsteps, every = h->rec_every;
+    const double dt = h->rec_dt;
+    const int nseg = (nsteps + every - 1) / every;
+    ca_status s;
+    float *Yw = abuf(h, AB_Y);
     if ((s = zero_d0(h, st))) return s;
     HIPCHK(h, hipMemcpyAsync(abuf(h, AB_UB), YbN, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
     const int64_t RS = rec_size(h);
+    // per segment (last first): re-run it from its checkpoint with stage records, then reverse its steps
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int len = std::min(every, nsteps - sg * every);
         HIPCHK(h, hipMemcpyAsync(Yw, h->aw_ck + (int64_t)sg * S, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice,
@@ -259,6 +309,36 @@ ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, doubl
     // the initial dss! and filter
     if ((s = dss_T_state(h, abuf(h, AB_UB), Yb0, 1, st))) return s;
     if (D0b && h->n.vdiff) return sum_d0(h, D0b, st);
+    return CA_OK;
+}
+
+ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, double dt, int32_t nsteps, int32_t every,
+                          float *Yb0, float *D0b, void *stream) {
+    if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
+    hipStream_t st = (hipStream_t)stream;
+    ca_status s;
+    if ((s = adj_ensure(h, every, (nsteps + every - 1) / every))) return s;
+    float *Yf = abuf(h, AB_YF);
+    HIPCHK(h, hipMemcpyAsync(Yf, Y0, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if ((s = ca_step_record(h, Yf, dt, nsteps, every, stream))) return s;
+    return ca_step_pullback(h, YbN, Yb0, D0b, stream);
+}
+
+ca_status ca_set_column_params(ca_handle *h, const double *alpha_w_col, const double *D0_col) {
+    if (!h) return CA_ERR_ARG;
+    const int nc = h->g.ncol;
+    if (alpha_w_col) {
+        if (!h->g.aw_col) return fail(h, CA_ERR_ARG, "ca_set_column_params: created without alpha_w_col");
+        std::vector<float> v(alpha_w_col, alpha_w_col + nc);
+        HIPCHK(h, hipMemcpy((void *)h->g.aw_col, v.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (D0_col) {
+        if (!h->g.D0_col) return fail(h, CA_ERR_ARG, "ca_set_column_params: created without vertical diffusion");
+        std::vector<float> v(D0_col, D0_col + nc);
+        HIPCHK(h, hipMemcpy((void *)h->g.D0_col, v.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    }
+    h->rec_valid = false;
     return CA_OK;
 }
 

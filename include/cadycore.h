@@ -138,12 +138,18 @@ ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *layouts
  *   CA_TUNE_DSS_KIND:       0 = staged through LDS per element cluster,
  *                           1 = direct gather from global memory.  Both sum
  *                           the same copies in the same order: results are
- *                           bitwise identical. */
+ *                           bitwise identical.
+ *   CA_TUNE_ADJ_CHUNK:      levels per work item of the explicit-tendency
+ *                           reverse (>= 1; the pullbacks' D0 partial sums
+ *                           follow it).
+ * The chunk defaults follow the grid size (enough work items to fill the
+ * GPU on small grids). */
 typedef enum ca_tune_knob {
     CA_TUNE_NEWTON = 0,
     CA_TUNE_EXPLICIT_CHUNK = 1,
     CA_TUNE_DSS_CHUNK = 2,
-    CA_TUNE_DSS_KIND = 3
+    CA_TUNE_DSS_KIND = 3,
+    CA_TUNE_ADJ_CHUNK = 4
 } ca_tune_knob;
 ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value);
 ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value);
@@ -231,14 +237,28 @@ ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream);
  * With vertical diffusion and D0b != NULL, D0b [ncol] <- d <Ytb, T_exp> / d D0. */
 ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, const float *Y, float *D0b,
                                     void *stream);
-/* The pullback of ca_step(h, Y0, dt, nsteps): given the cotangent YbN of the
+/* ca_step that also keeps what its pullback needs: runs nsteps steps of Y in
+ * place (as ca_step) and stores the state at every `every`-th step in the
+ * handle (ceil(nsteps / every) states; ca_step_pullback adds `every` steps of
+ * stage records, 7 states + 4 Laplacian sets each). */
+ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int32_t every, void *stream);
+/* The pullback of the last ca_step_record: given the cotangent YbN of its
+ * result, writes the cotangent of its input to Yb0 and, with vertical
+ * diffusion and D0b != NULL, each column's D0 cotangent to D0b [ncol].  Each
+ * checkpoint segment is re-run once with its stage states recorded, then
+ * reversed.  The column parameters must be those of the recorded run. */
+ca_status ca_step_pullback(ca_handle *h, const float *YbN, float *Yb0, float *D0b, void *stream);
+/* ca_step_record on a copy of Y0 followed by ca_step_pullback.
+ * The pullback of ca_step(h, Y0, dt, nsteps): given the cotangent YbN of the
  * state after nsteps steps from Y0, writes the cotangent of Y0 to Yb0 and,
  * with vertical diffusion and D0b != NULL, the cotangent of each column's D0
- * to D0b [ncol].  The forward trajectory is recomputed: checkpoints every
- * `every` steps, and each segment is re-run once with its stage states
- * recorded (device memory: (ceil(nsteps / every) + every * 8) states). */
+ * to D0b [ncol]. */
 ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, double dt, int32_t nsteps,
                           int32_t every, float *Yb0, float *D0b, void *stream);
+/* New per-column Rayleigh coefficients and / or vertical diffusivities
+ * (host arrays of ncol; NULL keeps), for handles created with them (the
+ * reference's parameter fields; calibration changes them between runs). */
+ca_status ca_set_column_params(ca_handle *h, const double *alpha_w_col, const double *D0_col);
 
 #ifdef __cplusplus
 }

@@ -260,3 +260,39 @@ def test_step_pullback_d0():
         rel = abs(gb[c] - fd) / max(abs(fd), 1e-300)
         print(f"col {c}: native {gb[c]:.8e} fd {fd:.8e} rel {rel:.2e}")
         assert rel <= 1e-4
+
+
+def test_c4_d0_gradient_matches_float64_fixture():
+    """The differentiation workload (h8 z30, 96 steps of dt = 150 s,
+    checkpoints every 8, squared column-integrated temperature loss) through
+    the autograd path (StepSolve: ca_step_record / ca_step_pullback): the D0
+    gradient on the fixture's columns against its float64 central
+    differences (tests/golden/make_c4_fd.py), relative L2 <= 1e-4 over the
+    columns and <= 1e-3 for each."""
+    torch = _torch()
+    path = os.path.join(HERE, "golden", "c4_fd_d0.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture tests/golden/c4_fd_d0.npz not generated")
+    from cadycore.adjoint import StepSolve
+    from cadycore.lib import DryDycore
+    from cadycore.workloads import c4_case, c4_loss
+
+    z = np.load(path)
+    n = int(z["done"])
+    cols, fd = z["cols"][:n], z["fd"][:n]
+    case = c4_case(PAR)
+    assert np.allclose(case["D0"][cols], z["D0"][:n])
+    d = DryDycore(case["grid"], PAR, case["numerics"], D0_col=case["D0"])
+    D0 = torch.tensor(case["D0"], dtype=torch.float64, requires_grad=True)
+    Y = StepSolve.apply(d.state(case["Y0"]), D0, d, case["dt"], case["nsteps"], case["every"])
+    L = c4_loss(Y.double(), case, xp=torch)
+    L.backward()
+    torch.cuda.synchronize()
+    g = D0.grad.numpy()[cols]
+    d.close()
+    rel = np.abs(g - fd) / np.abs(fd)
+    vec = np.linalg.norm(g - fd) / np.linalg.norm(fd)
+    print(f"loss {float(L):.8e} (float64 oracle {float(z['L0']):.8e}); {n} columns: relative L2 {vec:.2e}, "
+          f"max {rel.max():.2e}, median {np.median(rel):.2e}")
+    assert abs(float(L) / float(z["L0"]) - 1) < 1e-5
+    assert vec <= 1e-4 and rel.max() <= 1e-3

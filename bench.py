@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ragged column ensemble (512 columns, z_elem in {30, 45, 63, 90, 128}, 360 steps)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiation workload (h8 z30, 96 steps, D0 gradient of the column-integrated "
+                         "temperature loss)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
@@ -156,6 +159,55 @@ def ensemble_c3(torch, params):
     return out
 
 
+def adjoint_c4(torch, params):
+    """The differentiation workload (global h8 z30, 96 steps of 150 s,
+    checkpoints every 8): forward step, ca_step_record and the pullback of
+    the squared column-integrated temperature loss (D0 gradient), best of 3.
+    The adjoint/forward ratio is pullback / forward; the record costs a
+    forward."""
+    from cadycore.lib import DryDycore
+    from cadycore.workloads import c4_case, c4_loss
+
+    case = c4_case(params)
+    g, n = case["grid"], case["nsteps"]
+    d = DryDycore(g, params, case["numerics"], D0_col=case["D0"])
+    Y0 = d.state(case["Y0"])
+    Y = Y0.clone()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e9
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    tf = timed(lambda: d.step(Y.copy_(Y0), case["dt"], n))
+    tr = timed(lambda: d.step_record(Y.copy_(Y0), case["dt"], n, case["every"]))
+    Yd = Y.double().requires_grad_(True)
+    L = c4_loss(Yd, case, xp=torch)
+    L.backward()
+    YbN = Yd.grad.float().contiguous()
+    D0b = torch.zeros(g.ncol, dtype=torch.float32, device=Y.device)
+    tp = timed(lambda: d.step_pullback(YbN, D0b))
+    out = {
+        "workload": "dry global h_elem=8 Nq=4 z_elem=30, dt=150 s, 96 steps, checkpoints every 8, "
+                    "D0 gradient of mean (integral T dz - target)^2",
+        "forward_ms_per_step": round(tf / n * 1e3, 4),
+        "record_ms_per_step": round(tr / n * 1e3, 4),
+        "pullback_ms_per_step": round(tp / n * 1e3, 4),
+        "pullback_forward_ratio": round(tp / tf, 3),
+        "record_plus_pullback_forward_ratio": round((tr + tp) / tf, 3),
+        "loss": float(L.detach()),
+        "finite": bool(torch.isfinite(D0b).all().item()),
+    }
+    d.close()
+    return out
+
+
 def copy_calibration(torch):
     """Three launches of a plain device copy of 256 MiB (read 256 MiB, write
     256 MiB): the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -241,6 +293,7 @@ def main():
     if a.copy_calibration:
         copy_calibration(torch)
     ens = None if a.no_ensemble else ensemble_c3(torch, par)
+    adj = None if a.no_adjoint else adjoint_c4(torch, par)
 
     cpu = None
     if not a.no_cpu_baseline and a.cpu_baseline_steps > 0:
@@ -268,11 +321,12 @@ def main():
         "achieved_step_GBps": round(step_gbps, 1),
         "step_roofline_frac": round(step_gbps / HBM_PEAK_GBPS, 4),
         "steps_per_s": round(1e3 / ms, 2),
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": adj["pullback_forward_ratio"] if adj else None,
         "finite": bool(ok),
         "roofline": roof,
         "cpu_baseline": cpu,
         "c3_ensemble": ens,
+        "c4_adjoint": adj,
         "kernels": kernels,
     }
     print(json.dumps(line))

@@ -56,6 +56,21 @@ __device__ __forceinline__ CV cv_make(const DevGeom &g, const DevNum &n, int k, 
     return c;
 }
 
+// h_tot = e_tot + R_d T of a centre with cv_make's arithmetic (bitwise the
+// same h), without the pressure and Exner function it does not need
+__device__ __forceinline__ float cv_enthalpy(const DevGeom &g, const DevNum &n, int k, int col, float rho, float u1,
+                                             float u2, float rhoe, float u3a, float u3b) {
+    float ct1, ct2;
+    ca_contra(g, col, g.is2c[k], u1, u2, ct1, ct2);
+    const float vv = 0.5f * (u3a * g.g33f[k] * u3a + u3b * g.g33f[k + 1] * u3b);
+    const float K = 0.5f * ((u1 * ct1 + u2 * ct2) + vv);
+    const double e_tot = (double)rhoe * ca_drcp((double)rho);
+    const double e_int = e_tot - (double)K - g.Phic_d[k];
+    double Td = n.dT_0 + (e_int + n.dR_d * n.dT_0) * n.dicv_d;
+    Td = Td > n.dTmin ? Td : n.dTmin;
+    return (float)(e_tot + n.dR_d * Td);
+}
+
 __device__ __forceinline__ CV cv_load(const DevGeom &g, const DevNum &n, const float *Y, int k, int col) {
     const int64_t q = (int64_t)k * g.ncol + col;
     const float *u3 = Y + 4 * g.nC;

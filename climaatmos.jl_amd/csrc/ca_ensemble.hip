@@ -338,9 +338,8 @@ __device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg
     if (n.upw != 0) {
         for (int k = tid; k < nl; k += LPM) {
             const float u3a = k == 0 ? 0.0f : S.U[4][k], u3b = k + 1 >= nl ? 0.0f : S.U[4][k + 1];
-            const CV cn = cv_make(g, n, k, 0, S.U[0][k], S.U[1][k], S.U[2][k], S.U[3][k], u3a, u3b);
-            S.nw.hr[k] = cn.h;
-            S.nw.rJ[k] = cn.rho * cn.J;
+            S.nw.hr[k] = cv_enthalpy(g, n, k, 0, S.U[0][k], S.U[1][k], S.U[2][k], S.U[3][k], u3a, u3b);
+            S.nw.rJ[k] = S.U[0][k] * S.nw.c[NC_J][k];
         }
         sync();
         const float href = S.nw.hr[0];

@@ -194,10 +194,11 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         // post-Newton upwind correction at the new state
         if (n.upw != 0) {
             const float u3_np = pcr::dn(u3_n, 1);
-            const CV cn = cv_make(g, n, kc, col, rho_n, u1, u2, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
-            const float href = __shfl(cn.h, 0, 64);
-            const float hr = cn.h - href;
-            const float rJn = cn.rho * cn.J;
+            const float hn = cv_enthalpy(g, n, kc, col, rho_n, u1, u2, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
+            const float href = __shfl(hn, 0, 64);
+            const float hr = hn - href;
+            const float sk = g.sc[kc];
+            const float rJn = rho_n * (g.Jh[col] * sk * sk * g.Jvc[kc]);  // J: geometry only, reloaded
             const float h2 = pcr::up(hr, 2), h1 = pcr::up(hr, 1), h4 = pcr::dn(hr, 1);
             const float rJn1 = pcr::up(rJn, 1);
             float dF = 0.0f;
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
                 dF = rJf * upw - rJf * cen;
             }
             const float dF1 = pcr::dn(dF, 1);
-            const float corr = hasC ? -(dF1 - dF) * cn.iJ : 0.0f;
+            const float corr = hasC ? -(dF1 - dF) * (g.iJh[col] * g.isJc[kc]) : 0.0f;
             rhoe_n += dtg * corr;
             fi_e += corr;
         }

@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation workload (h8 z30, 96 steps, D0 gradient of the column-integrated "
                          "temperature loss)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the differentiation workload and print its figures (profiling runs)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the ragged column ensemble and print its figures (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
@@ -235,6 +239,10 @@ def main():
     from cadycore.lib import DryDycore
 
     par = P.Params.from_toml()
+    if a.adjoint_only or a.ensemble_only:
+        out = adjoint_c4(torch, par) if a.adjoint_only else ensemble_c3(torch, par)
+        print(json.dumps(out))
+        return 0
     zf = G.tanh_stretched_faces(a.z_elem, 30000.0, 30.0)
     grid = G.cubed_sphere(a.h_elem, zf, par.planet_radius, par.Omega)
     num = P.Numerics(dt=a.dt)

@@ -109,6 +109,17 @@ __device__ __forceinline__ float ca_dPhiref(const DevNum &n, float pa, float ipa
 __device__ __forceinline__ float ca_dExner(const DevNum &n, float pa, float ipa, float Pia, float pb) {
     return Pia * expm1f(n.kex * log1pf((pb - pa) * ipa));
 }
+// Both differences from one log1p and one expm1: with x = expm1(lr),
+// Pi_b - Pi_a = Pi_a x and expm1(7 lr) = (1 + x)^7 - 1 expanded in x (no
+// cancellation for small x).
+__device__ __forceinline__ void ca_dref(const DevNum &n, float pa, float ipa, float Pia, float pb, float &dPhr,
+                                        float &dPi) {
+    const float lr = n.kex * log1pf((pb - pa) * ipa);
+    const float x = expm1f(lr);
+    const float e7 = x * (7.0f + x * (21.0f + x * (35.0f + x * (35.0f + x * (21.0f + x * (7.0f + x))))));
+    dPi = Pia * x;
+    dPhr = -n.cp_d * (n.Tmr * lr + (n.Tsr - n.Tmr) * (1.0f / 7.0f) * ca_pow7(Pia) * e7);
+}
 
 // horizontal metric at scale s (no topography): contravariant u^a = g^{ab} u_b / s^2
 __device__ __forceinline__ void ca_contra(const DevGeom &g, int col, float is2, float a1, float a2, float &o1,

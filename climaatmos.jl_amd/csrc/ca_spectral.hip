@@ -158,8 +158,9 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 de -= 0.5f * wdh + 0.5f * (hh * wd + (m1 * gh1 + m2 * gh2)) + h0 * wd;
                 // u_h -= C12(grad(K + Phi - Phi_r) + cp_d (th' grad Pi + grad(th' Pi) - Pi grad th') / 2)
                 float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi), ip0 = ca_rcp(p0);
-                float A = K - ca_dPhiref(n, p0, ip0, Pi0, p);
-                float dPi = ca_dExner(n, p0, ip0, Pi0, p);
+                float dPhr, dPi;
+                ca_dref(n, p0, ip0, Pi0, p, dPhr, dPi);
+                float A = K - dPhr;
                 float thPi = T - ca_Tref(n, Pi);
                 float th = thPi * ca_rcp(Pi);
                 float g1, g2, a1, a2, b1, b2, c1, c2;

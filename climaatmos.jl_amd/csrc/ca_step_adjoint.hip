@@ -218,8 +218,9 @@ __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevN
     c.p0 = ca_elem0(c.p);
     c.Pi0 = ca_elem0(c.Pi);
     const float ip0 = ca_rcp(c.p0);
-    const float A = K - ca_dPhiref(n, c.p0, ip0, c.Pi0, c.p);
-    const float dPi = ca_dExner(n, c.p0, ip0, c.Pi0, c.p);
+    float dPhr, dPi;
+    ca_dref(n, c.p0, ip0, c.Pi0, c.p, dPhr, dPi);
+    const float A = K - dPhr;
     c.thPi = c.T - ca_Tref(n, c.Pi);
     c.th = c.thPi * ca_rcp(c.Pi);
     float g1, g2, b1, b2;

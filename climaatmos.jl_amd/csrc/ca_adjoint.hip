@@ -41,8 +41,9 @@ struct Cen {
     double rho, u1, u2, ct1, ct2, re, e, K, T, p, Pi;
     bool free_T;
 };
+// Td, pd, Pid: T, p and Pi of the same state as the Newton stage stored them
 __device__ __forceinline__ Cen cen_eval(const DevGeom &g, const DevNum &n, int k, double rho, double u1, double u2,
-                                        double re, double u3a, double u3b) {
+                                        double re, double u3a, double u3b, double Td, double pd, double Pid) {
     Cen c;
     c.rho = rho;
     c.u1 = u1;
@@ -53,11 +54,10 @@ __device__ __forceinline__ Cen cen_eval(const DevGeom &g, const DevNum &n, int k
     c.ct2 = is2 * ((double)g.gu12[0] * u1 + (double)g.gu22[0] * u2);
     c.K = 0.5 * ((u1 * c.ct1 + u2 * c.ct2) + 0.5 * ((double)g.g33f[k] * (u3a * u3a) + (double)g.g33f[k + 1] * (u3b * u3b)));
     c.e = re / rho;
-    const double Tu = n.dT_0 + n.dicv_d * (c.e - c.K + (-g.Phic_d[k] + n.dR_d * n.dT_0));
-    c.free_T = Tu > n.dTmin;
-    c.T = c.free_T ? Tu : n.dTmin;
-    c.p = n.dR_d * (rho * c.T);
-    c.Pi = exp(n.dkex * log(n.dp0inv * c.p));
+    c.T = Td;
+    c.free_T = Td > n.dTmin;  // the clamp T_min sets Td to exactly T_min
+    c.p = pd;
+    c.Pi = Pid;
     return c;
 }
 // reverse of cen_eval: cotangents of T, p (direct) and Pi -> rho, u1, u2, rho e_tot, u3a, u3b
@@ -86,12 +86,13 @@ struct FaceVals {
 // Face row f forward (float64) and its reverse seeded by ob = the
 // cotangents of (lo, d, up, r0, r1, e0, e1, Ru3): pb receives the
 // cotangents of the 11 inputs.  Returns the row's r0, r1, e0, e1.
+// tpa, tpb: (Td, pd, Pid) of the centres below and above, as stored.
 __device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, int f, float dtgf, const float *prim,
-                                             const double *ob, double *pb) {
+                                             const double *tpa, const double *tpb, const double *ob, double *pb) {
     const double dtg = dtgf;
     const double x8 = prim[8], x9 = prim[9], x10 = prim[10];
-    const Cen a = cen_eval(g, n, f - 1, prim[0], prim[1], prim[2], prim[3], x8, x9);
-    const Cen b = cen_eval(g, n, f, prim[4], prim[5], prim[6], prim[7], x9, x10);
+    const Cen a = cen_eval(g, n, f - 1, prim[0], prim[1], prim[2], prim[3], x8, x9, tpa[0], tpa[1], tpa[2]);
+    const Cen b = cen_eval(g, n, f, prim[4], prim[5], prim[6], prim[7], x9, x10, tpb[0], tpb[1], tpb[2]);
     const double kap = n.dR_d / n.dcv_d, cp = n.dcp_d, dTs = n.dTsr - n.dTmr;
     // ---- forward
     const double rf = 0.5 * (a.rho + b.rho);
@@ -718,7 +719,9 @@ __device__ double newton_vjp(AS &A, const DevGeom &g, const DevNum &n, float dtg
                                   Db * e0 + Ub * e1 + qb * Re,
                                   qb};
             double pbd[NP];
-            const FaceVals o = face_vjp(g, n, f, dtg, prim, ob, pbd);
+            const double tpa[3] = {S.nw.cd[0][f - 1], S.nw.cd[1][f - 1], S.nw.cd[2][f - 1]};
+            const double tpb[3] = {S.nw.cd[0][f], S.nw.cd[1][f], S.nw.cd[2][f]};
+            const FaceVals o = face_vjp(g, n, f, dtg, prim, tpa, tpb, ob, pbd);
 #pragma unroll
             for (int i = 0; i < NP; ++i) pb[i] = (float)pbd[i];
             if (f - 1 < 1) pb[8] = 0.0f;

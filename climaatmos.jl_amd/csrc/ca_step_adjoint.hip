@@ -644,21 +644,27 @@ template __global__ void k_explicit_vjp<false, false>(DevGeom, DevNum, const flo
                                                       float *, float *, float *, float *, double *, int);
 
 // ------------------------------------------------ implicit stage, per column
-// Reverse of one Newton stage (k_newton_pcr) on column blockIdx.x: given
-// U~ (Ut) and the cotangent Ub of the stage's result U, writes the cotangent
-// of U~ to Utb.  The column is viewed as a one-column grid (cg[col]: the
-// sphere's per-column planes offset to the column) and reversed by the
-// column ensemble's adj::newton_vjp, which re-runs the stage first.
-template <int LPM, int NF>
-__global__ void __launch_bounds__(LPM) __attribute__((amdgpu_waves_per_eu(4))) k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc,
-                                                    const float *__restrict__ Ut, const float *__restrict__ Ub,
-                                                    float dtg, float *__restrict__ Utb, const float *__restrict__ fib,
-                                                    float cfi, const float *__restrict__ fa,
-                                                    const float *__restrict__ fb) {
+// Reverse of one Newton stage (k_newton_pcr) on CPB columns per workgroup,
+// LPM lanes each: given U~ (Ut) and the cotangent Ub of the stage's result U,
+// writes the cotangent of U~ to Utb.  Each column is viewed as a one-column
+// grid (cg[col]: the sphere's per-column planes offset to the column) and
+// reversed by the column ensemble's adj::newton_vjp, which re-runs the stage
+// first.  With LPM = 32 two columns share a wave; a column index past the
+// grid's end (odd column counts) repeats the last column without writing, so
+// every lane reaches every barrier.
+template <int LPM, int NF, int CPB>
+__global__ void __launch_bounds__(LPM * CPB) __attribute__((amdgpu_waves_per_eu(4)))
+k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc, const float *__restrict__ Ut,
+             const float *__restrict__ Ub, float dtg, float *__restrict__ Utb, const float *__restrict__ fib, float cfi,
+             const float *__restrict__ fa, const float *__restrict__ fb) {
     using namespace adj;
-    __shared__ NewtonAdjShared<NF> A;
-    const int col = blockIdx.x;
-    const int tid = threadIdx.x;
+    __shared__ NewtonAdjShared<NF> As[CPB];
+    const int sub = CPB == 1 ? 0 : (int)threadIdx.x / LPM;
+    const int tid = CPB == 1 ? (int)threadIdx.x : (int)threadIdx.x % LPM;
+    const int64_t cwant = (int64_t)blockIdx.x * CPB + sub;
+    const bool live = cwant < nc;
+    const int64_t col = live ? cwant : nc - 1;
+    NewtonAdjShared<NF> &A = As[sub];
     const DevGeom &g = cg[col];
     const int nl = g.nlev;
     // the cotangent of U: Ub, + F_I's direct term -cfi fib (rho, rho e_tot,
@@ -683,6 +689,7 @@ __global__ void __launch_bounds__(LPM) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     __syncthreads();
     (void)newton_vjp<LPM>(A, g, n, dtg, A.Ut[0], A.S.fi[0], tid);
+    if (!live) return;
     // U~ also enters F_I = (U - U~) / dtg directly: Utb += cfi fib on rho, rho e_tot, u_3
     for (int k = tid; k < nl; k += LPM)
         for (int p = 0; p < 4; ++p) {
@@ -695,13 +702,13 @@ __global__ void __launch_bounds__(LPM) __attribute__((amdgpu_waves_per_eu(4))) k
     }
 }
 
-#define CA_NVJP_INST(LPM, NF)                                                                                        \
-    template __global__ void k_newton_vjp<LPM, NF>(const DevGeom *, DevNum, int64_t, int64_t, const float *,        \
-                                                   const float *, float, float *, const float *, float,          \
-                                                   const float *, const float *);
-CA_NVJP_INST(64, 33)
-CA_NVJP_INST(64, 65)
-CA_NVJP_INST(128, 129)
+#define CA_NVJP_INST(LPM, NF, CPB)                                                                                   \
+    template __global__ void k_newton_vjp<LPM, NF, CPB>(const DevGeom *, DevNum, int64_t, int64_t, const float *,   \
+                                                        const float *, float, float *, const float *, float,       \
+                                                        const float *, const float *);
+CA_NVJP_INST(32, 33, 2)
+CA_NVJP_INST(64, 65, 1)
+CA_NVJP_INST(128, 129, 1)
 #undef CA_NVJP_INST
 
 // ------------------------------------------------ stage algebra

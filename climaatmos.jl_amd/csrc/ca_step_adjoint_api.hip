@@ -152,16 +152,15 @@ const float *face_b(const ca_handle *h) { return h->aw_L + 12 * h->g.nC; }
 ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Ub, float dtg, float *Utb, const float *fib,
                      float cfi, const float *fa, const float *fb, hipStream_t st) {
     const DevGeom &g = h->g;
-    const dim3 grid(g.ncol);
     if (g.nlev <= 32)
-        sadj::k_newton_vjp<64, 33><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib, cfi, fa,
-                                                       fb);
+        sadj::k_newton_vjp<32, 33, 2><<<dim3(cdiv(g.ncol, 2)), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg,
+                                                                            Utb, fib, cfi, fa, fb);
     else if (g.nlev <= 64)
-        sadj::k_newton_vjp<64, 65><<<grid, 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib, cfi, fa,
-                                                       fb);
+        sadj::k_newton_vjp<64, 65, 1><<<dim3(g.ncol), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib,
+                                                                   cfi, fa, fb);
     else
-        sadj::k_newton_vjp<128, 129><<<grid, 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib, cfi,
-                                                          fa, fb);
+        sadj::k_newton_vjp<128, 129, 1><<<dim3(g.ncol), 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb,
+                                                                      fib, cfi, fa, fb);
     return check_launch(h, "k_newton_vjp");
 }
 

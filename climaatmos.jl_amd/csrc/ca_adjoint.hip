@@ -531,18 +531,50 @@ __device__ __forceinline__ void h_grad(const DevGeom &g, const DevNum &n, int k,
     out[5] += Kb * 0.5f * g.g33f[k + 1] * u3b;
 }
 
-// Reverse of one implicit stage.  On entry A.S.U = U~_i (also in Ut) and
-// A.Ub holds the cotangent of U_i; on exit A.Utb holds the cotangent of
-// U~_i.  Returns this thread's alpha_w cotangent.  Runs ens::newton_stage on
-// U~_i first to rebuild the stage's internals.
+// The internals of ens::newton_stage that its reverse reads, rebuilt from
+// the stage's recorded result instead of by re-solving the Schur system.
+// On entry S.U holds U~, and rhoN, u3N (LDS) rho and u_3 of the recorded
+// result U.  The face increment is x = u~_3 - u_3: what the stage subtracted,
+// up to the rounding of that float subtraction (exact when the two are
+// within a factor 2).  rho e_tot before the upwind correction is
+// (rho e_tot)~ - de(x) with de = e0 x_k + e1 x_k+1 - Re, as in the stage.
+// On exit S.U holds U with rho e_tot before the correction, and S.nw.c, cd,
+// row, x, ve (and hr, rJ with upwinding) hold what newton_stage leaves.
+template <int LPM, class SH>
+__device__ void newton_replay(SH &S, const DevGeom &g, const DevNum &n, float dtg, const float *rhoN,
+                              const float *u3N, int tid) {
+    const int nl = g.nlev;
+    ens::stage_thermo<LPM>(S, g, n, tid);
+    for (int f = tid; f <= nl; f += LPM) S.nw.x[f] = (f >= 1 && f <= nl - 1) ? S.U[4][f] - u3N[f] : 0.0f;
+    sync();
+    ens::stage_rows<LPM>(S, g, dtg, tid);
+    sync();
+    for (int k = tid; k < nl; k += LPM) {
+        const float de = S.nw.row[2][k] * S.nw.x[k] + S.nw.row[3][k] * S.nw.x[k + 1] - S.nw.row[5][k];
+        S.U[0][k] = rhoN[k];
+        S.U[3][k] -= de;
+        S.nw.ve[k] = S.U[3][k];
+    }
+    for (int f = tid; f <= nl; f += LPM) S.U[4][f] = (f >= 1 && f <= nl - 1) ? u3N[f] : 0.0f;
+    sync();
+    if (n.upw != 0) {
+        ens::stage_enthalpy<LPM>(S, g, n, tid);
+        sync();
+    }
+}
+
+// Reverse of one implicit stage (ens::newton_stage) given its recorded
+// result (rhoN, u3N: see newton_replay).  On entry S.U holds U~ (= Ut) and
+// A.Ub the cotangent of the result U; on exit A.Utb holds that of U~.
+// Returns this thread's share of the alpha_w cotangent.
 template <int LPM, class AS>
 __device__ double newton_vjp(AS &A, const DevGeom &g, const DevNum &n, float dtg, const float (*Ut)[AS::NFv],
-                            float (*fi)[AS::NFv], int tid) {
+                            const float *rhoN, const float *u3N, int tid) {
     constexpr int NF = AS::NFv;
     auto &S = A.S;
     const int nl = g.nlev;
     float (*W)[NF] = A.W;
-    ens::newton_stage<LPM>(S, g, n, dtg, fi, tid);  // S.U = U_i, S.nw = internals; V_i (rho e) in S.nw.ve
+    newton_replay<LPM>(S, g, n, dtg, rhoN, u3N, tid);  // S.U = U_i, S.nw = internals; V_i (rho e) in S.nw.ve
     double awbar = 0.0;
     // ---- upwind correction: U_e = V_e + dtg corr(V); V = U except rho e
     if (n.upw != 0) {
@@ -899,7 +931,7 @@ __device__ void reverse_step(AdjSharedT<NF> &A, const DevGeom &g, const DevNum &
             A.S.U[p][k] = A.Ut[i - 1][p][k];
         }
         sync();
-        ab += newton_vjp<LPM>(A, g, n, dtg, A.Ut[i - 1], A.S.fi[i - 1], tid);
+        ab += newton_vjp<LPM>(A, g, n, dtg, A.Ut[i - 1], A.Us[i - 1][0], A.Us[i - 1][4], tid);
         // U~_i = u + dt sum_j<i (aE F_E_j + aI F_I_j)
         for (int q = tid; q < 5 * NF; q += LPM) {
             const int p = q / NF, k = q - p * NF;

@@ -199,16 +199,11 @@ __device__ void explicit_tendency(SharedT<NF> &S, const DevGeom &g, const DevNum
     sync();
 }
 
-// One implicit stage: S.U holds U~ on entry and U on exit; fi receives
-// F_I = (U - U~) / dtg for rho, rho e_tot and u_3.  Same algorithm as
-// k_newton_pcr (one Newton iteration from U~, manual Jacobian, Schur
-// complement on u_3, post-Newton upwind correction).
+// Thermodynamic state of the stage state U~ in S.U, centre by centre, into
+// S.nw.c and S.nw.cd.
 template <int LPM, class SH>
-__device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg, float (*fi)[SH::NFv],
-                             int tid) {
+__device__ __forceinline__ void stage_thermo(SH &S, const DevGeom &g, const DevNum &n, int tid) {
     const int nl = g.nlev;
-    const float idtg = 1.0f / dtg;
-    // centres: thermodynamic state of U~
     for (int k = tid; k < nl; k += LPM) {
         const float u3a = k == 0 ? 0.0f : S.U[4][k], u3b = k + 1 >= nl ? 0.0f : S.U[4][k + 1];
         const CV c = cv_make(g, n, k, 0, S.U[0][k], S.U[1][k], S.U[2][k], S.U[3][k], u3a, u3b);
@@ -227,8 +222,13 @@ __device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg
         S.nw.cd[1][k] = c.pd;
         S.nw.cd[2][k] = c.Pid;
     }
-    sync();
-    // centre rows: A(rho, u3), A(rho e, u3) on faces k, k+1 and R = dtg T_imp
+}
+
+// Centre rows of the Newton system at U~: A(rho, u3), A(rho e, u3) on faces
+// k, k+1 and R = dtg T_imp, into S.nw.row (reads S.nw.c).
+template <int LPM, class SH>
+__device__ __forceinline__ void stage_rows(SH &S, const DevGeom &g, float dtg, int tid) {
+    const int nl = g.nlev;
     for (int k = tid; k < nl; k += LPM) {
         const float rJ = S.nw.c[NC_RJ][k], h = S.nw.c[NC_H][k], iJ = S.nw.c[NC_IJ][k];
         float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f, Fm = 0.0f, Fp = 0.0f, Hm = 0.0f, Hp = 0.0f;
@@ -257,6 +257,34 @@ __device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg
         S.nw.row[4][k] = dtg * (-(Fp - Fm) * iJ);
         S.nw.row[5][k] = dtg * (-(Hp - Hm) * iJ);
     }
+}
+
+// Enthalpy relative to the lowest cell and rho J of the post-Newton state in
+// S.U, the inputs of the upwind correction, into S.nw.hr and S.nw.rJ.
+template <int LPM, class SH>
+__device__ __forceinline__ void stage_enthalpy(SH &S, const DevGeom &g, const DevNum &n, int tid) {
+    const int nl = g.nlev;
+    for (int k = tid; k < nl; k += LPM) {
+        const float u3a = k == 0 ? 0.0f : S.U[4][k], u3b = k + 1 >= nl ? 0.0f : S.U[4][k + 1];
+        S.nw.hr[k] = cv_enthalpy(g, n, k, 0, S.U[0][k], S.U[1][k], S.U[2][k], S.U[3][k], u3a, u3b);
+        S.nw.rJ[k] = S.U[0][k] * S.nw.c[NC_J][k];
+    }
+}
+
+// One implicit stage: S.U holds U~ on entry and U on exit; fi receives
+// F_I = (U - U~) / dtg for rho, rho e_tot and u_3.  Same algorithm as
+// k_newton_pcr (one Newton iteration from U~, manual Jacobian, Schur
+// complement on u_3, post-Newton upwind correction).
+template <int LPM, class SH>
+__device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg, float (*fi)[SH::NFv],
+                             int tid) {
+    const int nl = g.nlev;
+    const float idtg = 1.0f / dtg;
+    // centres: thermodynamic state of U~
+    stage_thermo<LPM>(S, g, n, tid);
+    sync();
+    // centre rows: A(rho, u3), A(rho e, u3) on faces k, k+1 and R = dtg T_imp
+    stage_rows<LPM>(S, g, dtg, tid);
     sync();
     // face rows of the Schur system, faces 0 .. nl - 1 (face nl: u_3 = 0, x = 0)
     for (int f = tid; f < nl; f += LPM) {
@@ -336,11 +364,7 @@ __device__ void newton_stage(SH &S, const DevGeom &g, const DevNum &n, float dtg
     sync();
     // post-Newton upwind correction at the new state (implicit_tendency.jl:322-339)
     if (n.upw != 0) {
-        for (int k = tid; k < nl; k += LPM) {
-            const float u3a = k == 0 ? 0.0f : S.U[4][k], u3b = k + 1 >= nl ? 0.0f : S.U[4][k + 1];
-            S.nw.hr[k] = cv_enthalpy(g, n, k, 0, S.U[0][k], S.U[1][k], S.U[2][k], S.U[3][k], u3a, u3b);
-            S.nw.rJ[k] = S.U[0][k] * S.nw.c[NC_J][k];
-        }
+        stage_enthalpy<LPM>(S, g, n, tid);
         sync();
         const float href = S.nw.hr[0];
         for (int f = tid; f <= nl; f += LPM) {

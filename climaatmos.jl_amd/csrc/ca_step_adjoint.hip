@@ -645,18 +645,18 @@ template __global__ void k_explicit_vjp<false, false>(DevGeom, DevNum, const flo
 
 // ------------------------------------------------ implicit stage, per column
 // Reverse of one Newton stage (k_newton_pcr) on CPB columns per workgroup,
-// LPM lanes each: given U~ (Ut) and the cotangent Ub of the stage's result U,
-// writes the cotangent of U~ to Utb.  Each column is viewed as a one-column
-// grid (cg[col]: the sphere's per-column planes offset to the column) and
-// reversed by the column ensemble's adj::newton_vjp, which re-runs the stage
-// first.  With LPM = 32 two columns share a wave; a column index past the
+// LPM lanes each: given U~ (Ut), the stage's recorded result U (Un) and the
+// cotangent Ub of U, writes the cotangent of U~ to Utb.  Each column is
+// viewed as a one-column grid (cg[col]: the sphere's per-column planes
+// offset to the column) and reversed by the column ensemble's
+// adj::newton_vjp, which rebuilds the stage's internals from U~ and U.  With LPM = 32 two columns share a wave; a column index past the
 // grid's end (odd column counts) repeats the last column without writing, so
 // every lane reaches every barrier.
 template <int LPM, int NF, int CPB>
 __global__ void __launch_bounds__(LPM * CPB) __attribute__((amdgpu_waves_per_eu(4)))
 k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc, const float *__restrict__ Ut,
-             const float *__restrict__ Ub, float dtg, float *__restrict__ Utb, const float *__restrict__ fib, float cfi,
-             const float *__restrict__ fa, const float *__restrict__ fb) {
+             const float *__restrict__ Un, const float *__restrict__ Ub, float dtg, float *__restrict__ Utb,
+             const float *__restrict__ fib, float cfi, const float *__restrict__ fa, const float *__restrict__ fb) {
     using namespace adj;
     __shared__ NewtonAdjShared<NF> As[CPB];
     const int sub = CPB == 1 ? 0 : (int)threadIdx.x / LPM;
@@ -676,6 +676,7 @@ k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc, c
             A.S.U[p][k] = v;
             A.Ut[0][p][k] = v;
             A.Ub[p][k] = (p == 0 || p == 3) ? fmaf(-cfi, fib[i], Ub[i]) : Ub[i];
+            if (p == 0) A.W[22][k] = Un[i];
         }
     for (int f = tid; f <= nl; f += LPM) {
         const bool in = f >= 1 && f <= nl - 1;
@@ -683,12 +684,13 @@ k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc, c
         const float v = in ? Ut[i] : 0.0f;
         A.S.U[4][f] = v;
         A.Ut[0][4][f] = v;
+        A.W[23][f] = in ? Un[i] : 0.0f;
         float b = in ? fmaf(-cfi, fib[i], Ub[i]) : 0.0f;
         if (in && fa) b += fa[q] + fb[q - nc];
         A.Ub[4][f] = b;
     }
     __syncthreads();
-    (void)newton_vjp<LPM>(A, g, n, dtg, A.Ut[0], A.S.fi[0], tid);
+    (void)newton_vjp<LPM>(A, g, n, dtg, A.Ut[0], A.W[22], A.W[23], tid);  // W[22], W[23]: rho, u_3 of U
     if (!live) return;
     // U~ also enters F_I = (U - U~) / dtg directly: Utb += cfi fib on rho, rho e_tot, u_3
     for (int k = tid; k < nl; k += LPM)
@@ -704,8 +706,8 @@ k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc, c
 
 #define CA_NVJP_INST(LPM, NF, CPB)                                                                                   \
     template __global__ void k_newton_vjp<LPM, NF, CPB>(const DevGeom *, DevNum, int64_t, int64_t, const float *,   \
-                                                        const float *, float, float *, const float *, float,       \
-                                                        const float *, const float *);
+                                                        const float *, const float *, float, float *,              \
+                                                        const float *, float, const float *, const float *);
 CA_NVJP_INST(32, 33, 2)
 CA_NVJP_INST(64, 65, 1)
 CA_NVJP_INST(128, 129, 1)

@@ -145,22 +145,22 @@ ca_status explicit_vjp(ca_handle *h, const float *U, const float *L, const float
 const float *face_a(const ca_handle *h) { return h->aw_L + 8 * h->g.nC; }
 const float *face_b(const ca_handle *h) { return h->aw_L + 12 * h->g.nC; }
 
-// Utb = (d Newton_i / d U~)^T Ub at U~ = Ut, column by column, + cfi fib
-// on rho, rho e_tot and u_3
+// Utb = (d Newton_i / d U~)^T Ub at U~ = Ut (Un: the stage's recorded
+// result), column by column, + cfi fib on rho, rho e_tot and u_3
 // The cotangent of U is Ub - cfi fib (rho, rho e_tot, u_3) + the face terms
 // fa, fb (if given): F_I = (U - U~) / dtg with cfi = -1 / dtg.
-ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Ub, float dtg, float *Utb, const float *fib,
-                     float cfi, const float *fa, const float *fb, hipStream_t st) {
+ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Un, const float *Ub, float dtg, float *Utb,
+                     const float *fib, float cfi, const float *fa, const float *fb, hipStream_t st) {
     const DevGeom &g = h->g;
     if (g.nlev <= 32)
-        sadj::k_newton_vjp<32, 33, 2><<<dim3(cdiv(g.ncol, 2)), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg,
-                                                                            Utb, fib, cfi, fa, fb);
+        sadj::k_newton_vjp<32, 33, 2><<<dim3(cdiv(g.ncol, 2)), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Un, Ub,
+                                                                            dtg, Utb, fib, cfi, fa, fb);
     else if (g.nlev <= 64)
-        sadj::k_newton_vjp<64, 65, 1><<<dim3(g.ncol), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb, fib,
-                                                                   cfi, fa, fb);
+        sadj::k_newton_vjp<64, 65, 1><<<dim3(g.ncol), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Un, Ub, dtg, Utb,
+                                                                   fib, cfi, fa, fb);
     else
-        sadj::k_newton_vjp<128, 129, 1><<<dim3(g.ncol), 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Ub, dtg, Utb,
-                                                                      fib, cfi, fa, fb);
+        sadj::k_newton_vjp<128, 129, 1><<<dim3(g.ncol), 128, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Un, Ub, dtg,
+                                                                      Utb, fib, cfi, fa, fb);
     return check_launch(h, "k_newton_vjp");
 }
 
@@ -201,7 +201,8 @@ ca_status reverse_step(ca_handle *h, float *rec, double dt, hipStream_t st) {
         if ((s = explicit_vjp(h, rec_U(h, rec, i), rec_L(h, rec, i), gb, Ubs, st, false))) return s;
         // U_i = Newton(U~_i); F_I_i = (U_i - U~_i) / dtg and the Laplacian
         // face terms are added to the U_i cotangent as the kernel loads it
-        if ((s = newton_vjp(h, rec_Ut(h, rec, i), Ubs, (float)dtg, Utb[i], fib, (float)(-1.0 / dtg), fa, fb, st)))
+        if ((s = newton_vjp(h, rec_Ut(h, rec, i), rec_U(h, rec, i), Ubs, (float)dtg, Utb[i], fib, (float)(-1.0 / dtg),
+                            fa, fb, st)))
             return s;
     }
     // F_E_0 = DSS(T_E(u))

@@ -69,6 +69,17 @@ struct DevNum {
     int hyper, upw, rayleigh, vdiff, sphere;
 };
 
+// ------------------------------------------------------------- launch order
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b mod 8), each XCD with its own L2.  ca_xcd_tile maps workgroup b of nb to
+// a tile index so that every XCD takes one contiguous range of tiles, in
+// order: tiles that share cache lines (narrow column tiles of a level-major
+// plane) are then fetched through the same L2.  A bijection of [0, nb).
+__device__ __forceinline__ int ca_xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
+    return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
 // ------------------------------------------------------------- thermodynamics
 // Dry air (Thermodynamics 1.3.0 as used at precomputed_quantities.jl:786-815):
 // e_int = cv_d (T - T_0) - R_d T_0, p = rho R_d T, h_tot = e_tot + R_d T,

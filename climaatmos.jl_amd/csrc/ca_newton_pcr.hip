@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
     __shared__ float tF[3][64][R];
     const int nl = g.nlev;
     const int64_t nC = g.nC, nc = g.ncol;
-    const int col0 = blockIdx.x * TC;
+    const int col0 = ca_xcd_tile(blockIdx.x, gridDim.x) * TC;
     const int tid = threadIdx.x;
 
     // ---- phase 1: assemble U~ (coalesced rows of TC columns)

@@ -661,7 +661,7 @@ k_newton_vjp(const DevGeom *__restrict__ cg, DevNum n, int64_t nC, int64_t nc, c
     __shared__ NewtonAdjShared<NF> As[CPB];
     const int sub = CPB == 1 ? 0 : (int)threadIdx.x / LPM;
     const int tid = CPB == 1 ? (int)threadIdx.x : (int)threadIdx.x % LPM;
-    const int64_t cwant = (int64_t)blockIdx.x * CPB + sub;
+    const int64_t cwant = (int64_t)ca_xcd_tile(blockIdx.x, gridDim.x) * CPB + sub;
     const bool live = cwant < nc;
     const int64_t col = live ? cwant : nc - 1;
     NewtonAdjShared<NF> &A = As[sub];

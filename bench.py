@@ -26,9 +26,10 @@ sys.path.insert(0, os.path.join(ROOT, "climaatmos.jl_amd"))
 
 METRIC = "SYPD + achieved HBM GB/s, dry global h30 z63 Nq4 FP32; adjoint/forward ratio"
 HBM_PEAK_GBPS = 8000.0
-# bytes per state float moved by an ideal fused ARS343 step (compulsory
-# traffic; 58.2 state sizes per step)
-B_ALG_STATES_PER_STEP = 58.2
+# launches per ARS343 step of each kernel of kernel_bytes (the step's
+# compulsory traffic is their weighted sum: 57.2 state sizes at h30 z63)
+LAUNCHES_PER_STEP = {"k_laplacians": 4, "k_dss(laplacians)": 4, "k_explicit": 4, "k_dss(tendency)": 4,
+                     "k_newton_stage": 3, "k_final": 1, "k_dss(state)": 1}
 
 
 def parse():
@@ -62,7 +63,8 @@ def kernel_bytes(S, nC, nF):
     (rho, rho e_tot and u_3 planes).  Newton stage i (i = 1, 2, 3) reads u,
     i explicit and i - 1 implicit tendencies and writes U and F_I; the three
     stages are averaged, as their launches are.  The final combination reads
-    u, 4 explicit and 3 implicit tendencies and writes u."""
+    u, the explicit tendencies of stages 1..3 (ARS343's b_0 is 0, so F_E_0 is
+    not read) and 3 implicit tendencies and writes u."""
     fi = 2 * nC + nF
     newton = sum((S + i * S + (i - 1) * fi) + (S + fi) for i in (1, 2, 3)) / 3.0
     return {
@@ -71,7 +73,7 @@ def kernel_bytes(S, nC, nF):
         "k_explicit": 4 * (S + 4 * nC + S),
         "k_dss(tendency)": 4 * (2 * S),
         "k_newton_stage": 4 * newton,
-        "k_final": 4 * (S + 4 * S + 3 * fi + S),
+        "k_final": 4 * (S + 3 * S + 3 * fi + S),
         "k_dss(state)": 4 * (2 * S),
     }
 
@@ -262,7 +264,7 @@ def main():
     ok = dyc.check_finite(Y)
     ms = el / a.steps * 1e3
     sypd = (a.dt / (el / a.steps)) / 365.0
-    b_alg = B_ALG_STATES_PER_STEP * S * 4
+    b_alg = sum(LAUNCHES_PER_STEP[k] * v for k, v in kernel_bytes(S, nC, nF).items())
     step_gbps = b_alg / (el / a.steps) / 1e9
 
     # per-kernel device time with HIP events on the launch stream (separate, untimed pass)

@@ -18,20 +18,22 @@
 // reused over the thread's levels.
 #include "ca_kernels.h"
 
-// The 1-D grid covers (column block, level chunk) pairs.  Blocks are
-// dispatched round-robin over the 8 XCDs; the logical order is remapped to
-// give every XCD a contiguous span of column blocks, so neighbouring
-// elements, whose copies a gather reads, share that XCD's L2.  The host pads
-// the grid to a multiple of 8 blocks: b -> (b % 8) * q + b / 8 is then a
-// bijection onto [0, 8 q), and logical blocks past the end return.
+// The 1-D grid covers (column block, level chunk) pairs, level chunk major.
+// Blocks are dispatched round-robin over the 8 XCDs; the logical order is
+// remapped to give every XCD a contiguous span of logical blocks: column
+// blocks of one level chunk, in order, so neighbouring elements, whose
+// copies a gather reads, are fetched at the same levels through that XCD's
+// L2 at about the same time.  The host pads the grid to a multiple of 8
+// blocks: b -> (b % 8) * q + b / 8 is then a bijection onto [0, 8 q), and
+// logical blocks past the end return.
 __device__ __forceinline__ bool xcd_block(int nb_col, int nb_lev, int &cb, int &lb) {
     const int nb = nb_col * nb_lev;
     const int q = (nb + 7) / 8;
     const int b = blockIdx.x;
     const int logical = (b & 7) * q + (b >> 3);
     if (logical >= nb) return false;
-    cb = logical / nb_lev;
-    lb = logical - cb * nb_lev;
+    lb = logical / nb_col;
+    cb = logical - lb * nb_col;
     return true;
 }
 

@@ -174,9 +174,13 @@ __device__ __forceinline__ Nod ca_node(int lane) {
     }
     return q;
 }
+// Every pattern used here (quad_perm, row_ror, row_newbcast with full row and
+// bank masks) has a valid source lane for every lane, so bound_ctrl changes
+// no value; it lets the compiler drop the zero "old" operand it would
+// otherwise materialise before each move.
 template <int CTRL>
 __device__ __forceinline__ float ca_dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 // value at node (0, 0) of the element (DPP row_newbcast:0)
 __device__ __forceinline__ float ca_elem0(float v) { return ca_dpp<0x150>(v); }

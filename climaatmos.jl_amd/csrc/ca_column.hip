@@ -46,7 +46,7 @@ __device__ __forceinline__ CV cv_make(const DevGeom &g, const DevNum &n, int k, 
     double pd = (double)rho * n.dR_d * Td;
     c.Td = Td;
     c.pd = pd;
-    c.Pid = exp(n.dkex * log(pd * n.dp0inv));
+    c.Pid = ca_dexner(n, pd * n.dp0inv);
     c.T = (float)Td;
     c.h = (float)(e_tot + n.dR_d * Td);
     c.p = (float)pd;

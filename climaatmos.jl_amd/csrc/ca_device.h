@@ -64,6 +64,7 @@ struct DevNum {
     float R_d, cp_d, cv_d, T_0, p0inv, kex, kap, grav, Tsr, Tmr, Tmin;
     double dR_d, dcp_d, dcv_d, dT_0, dp0inv, dkex, dTsr, dTmr, dTmin;
     double dicv_d;  // 1 / cv_d
+    double dkex_r;  // kex - 2/7 (|.| <= 1e-6, checked at create)
     float nu_v, nu_s, dd;
     float alpha_uh, alpha_w, dt;
     int hyper, upw, rayleigh, vdiff, sphere;
@@ -100,6 +101,23 @@ __device__ __forceinline__ double ca_drcp(double x) {
     r = fma(fma(-x, r, 1.0), r, r);
     r = fma(fma(-x, r, 1.0), r, r);
     return r;
+}
+// Exner function q^kex in float64 for q = p / p0 > 0, without a float64
+// log or exp.  The dry-air exponent is 2/7 up to kex_r = kex - 2/7 (rounding
+// of the parameter set's kappa_d; |kex_r| <= 1e-6 is checked at create):
+//   y0 = q^(2/7) from the hardware log2 / exp2 (relative error ~2e-7);
+//   one Newton step on y^7 = q^2: y = (6 y0 + q^2 / y0^6) / 7 (relative
+//   error ~3 e0^2 ~ 1e-13);
+//   q^kex = y q^kex_r = y (1 + kex_r ln q + O((kex_r ln q)^2)), where
+//   (kex_r ln q)^2 <= 1e-10 for q >= 1e-3 and 0 for an exact 2/7.
+__device__ __forceinline__ double ca_dexner(const DevNum &n, double q) {
+    const float qf = (float)q;
+    const float l2 = __builtin_amdgcn_logf(qf);  // log2 q
+    const double y0 = (double)__builtin_amdgcn_exp2f((2.0f / 7.0f) * l2);
+    const double y2 = y0 * y0;
+    const double y6 = y2 * y2 * y2;
+    const double y = (6.0 * y0 + (q * q) * ca_drcp(y6)) * (1.0 / 7.0);
+    return fma(y, n.dkex_r * (0.6931471805599453 * (double)l2), y);
 }
 // reference temperature T_r(Pi) = T_min + (T_sfc - T_min) Pi^7 (refstate_thermodynamics.jl:80-88)
 __device__ __forceinline__ float ca_pow7(float x) {

@@ -110,6 +110,10 @@ ca_status ens_create_impl(ca_ensemble *e, const ca_params *par, const ca_config 
     n.dT_0 = par->T_0;
     n.dp0inv = 1.0 / par->p_ref_theta;
     n.dkex = par->R_d / par->cp_d;
+    n.dkex_r = n.dkex - 2.0 / 7.0;
+    if (std::fabs(n.dkex_r) > 1e-6)
+        return efail(e, CA_ERR_ARG, "ca_ensemble_create: R_d / cp_d must be 2/7 to within 1e-6 (dry air; the float64 "
+                                    "Exner function refines q^(2/7))");
     n.dTsr = par->T_surf_ref;
     n.dTmr = par->T_min_ref;
     n.dTmin = par->T_min_sgs;

@@ -75,7 +75,9 @@ typedef struct ca_geometry {
     const double *gll_D;    /* [16] D[i*4+k] = l_k'(xi_i)                    */
 } ca_geometry;
 
-/* Thermodynamic constants (ClimaParams names in cadycore/ca_params.toml). */
+/* Thermodynamic constants (ClimaParams names in cadycore/ca_params.toml).
+ * Dry air: R_d / cp_d (kappa_d) must be 2/7 to within 1e-6, else ca_create
+ * and ca_ensemble_create return CA_ERR_ARG. */
 typedef struct ca_params {
     double R_d, cp_d, cv_d, T_0, p_ref_theta, grav, T_surf_ref, T_min_ref, T_min_sgs;
 } ca_params;

@@ -134,10 +134,6 @@ __device__ __forceinline__ float ca_dPhiref(const DevNum &n, float pa, float ipa
     float lr = n.kex * log1pf((pb - pa) * ipa);
     return -n.cp_d * (n.Tmr * lr + (n.Tsr - n.Tmr) * (1.0f / 7.0f) * ca_pow7(Pia) * expm1f(7.0f * lr));
 }
-// Pi_b - Pi_a = Pi_a expm1(kex ln(p_b / p_a))
-__device__ __forceinline__ float ca_dExner(const DevNum &n, float pa, float ipa, float Pia, float pb) {
-    return Pia * expm1f(n.kex * log1pf((pb - pa) * ipa));
-}
 // Both differences from one log1p and one expm1: with x = expm1(lr),
 // Pi_b - Pi_a = Pi_a x and expm1(7 lr) = (1 + x)^7 - 1 expanded in x (no
 // cancellation for small x).

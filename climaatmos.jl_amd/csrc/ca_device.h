@@ -160,6 +160,30 @@ __device__ __forceinline__ void ca_cov(const DevGeom &g, int col, float s2, floa
     o2 = s2 * (g12 * a1 + g22 * a2);
 }
 
+// The horizontal metric of one column (level independent), loaded once by
+// kernels that loop over levels, so the per-level arithmetic reads registers.
+struct ColMet {
+    float gu11, gu12, gu22, gd11, gd12, gd22;
+};
+__device__ __forceinline__ ColMet ca_colmet(const DevGeom &g, int col) {
+    ColMet c;
+    c.gu11 = g.gu11[col];
+    c.gu12 = g.gu12[col];
+    c.gu22 = g.gu22[col];
+    c.gd11 = g.gd11[col];
+    c.gd12 = g.gd12[col];
+    c.gd22 = g.gd22[col];
+    return c;
+}
+__device__ __forceinline__ void ca_contra(const ColMet &c, float is2, float a1, float a2, float &o1, float &o2) {
+    o1 = is2 * (c.gu11 * a1 + c.gu12 * a2);
+    o2 = is2 * (c.gu12 * a1 + c.gu22 * a2);
+}
+__device__ __forceinline__ void ca_cov(const ColMet &c, float s2, float a1, float a2, float &o1, float &o2) {
+    o1 = s2 * (c.gd11 * a1 + c.gd12 * a2);
+    o2 = s2 * (c.gd12 * a1 + c.gd22 * a2);
+}
+
 // ------------------------------------------------- element-local derivatives
 // The 16 nodes of an element occupy one 16-lane DPP row of a wavefront
 // (lane = 16 e + 4 j + i).  Along the first reference coordinate the four

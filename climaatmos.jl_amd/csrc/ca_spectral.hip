@@ -105,6 +105,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
     const float W = c_W[q.j * 4 + q.i], iW = c_iW[q.j * 4 + q.i];
     const float Jh = g.Jh[col], iJh = g.iJh[col];
     const float f3n = g.f3n[col], f12a = g.f12a[col], f12b = g.f12b[col];
+    const ColMet cm = ca_colmet(g, col);
     float Kf = 0.0f;  // face diffusivity: harmonic mean of the (equal) centre values
     if (VDIFF) {
         const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             const Met m = ca_met(W, iW, Jh * s2 * Jv, iJh * g.isJc[k]);
             const float J = m.J;
             float ct1, ct2;
-            ca_contra(g, col, is2, u1, u2, ct1, ct2);
+            ca_contra(cm, is2, u1, u2, ct1, ct2);
             float ga = g.g33f[k], gb = g.g33f[k + 1];
             const float Khor = 0.5f * (u1 * ct1 + u2 * ct2);
             const float Kver = 0.25f * (u3a * ga * u3a + u3b * gb * u3b);
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 if (HYPER) {
                     float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
                     float v1, v2;
-                    ca_contra(g, col, is2, X1, X2, v1, v2);
+                    ca_contra(cm, is2, X1, X2, v1, v2);
                     float dv = op_div(q, m, v1, v2);
                     float gd1, gd2;
                     op_wgrad(q, m, dv, gd1, gd2);
@@ -182,13 +183,13 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     float cr1, cr2;
                     op_curl12(q, m, X3, cr1, cr2);
                     float cc1, cc2;
-                    ca_cov(g, col, s2, cr1, cr2, cc1, cc2);
+                    ca_cov(cm, s2, cr1, cr2, cc1, cc2);
                     float cc3 = Jv * Jv * cr3;
                     float w3 = op_wcurl3(q, m, cc1, cc2);
                     float w1, w2;
                     op_wcurl12(q, m, cc3, w1, w2);
                     float wc1, wc2;
-                    ca_cov(g, col, s2, w1, w2, wc1, wc2);
+                    ca_cov(cm, s2, w1, w2, wc1, wc2);
                     float wc3 = Jv * Jv * w3;
                     d1 -= n.nu_v * (n.dd * gd1 - wc1);
                     d2 -= n.nu_v * (n.dd * gd2 - wc2);
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     float s1, s2g;
                     op_grad(q, Ls, s1, s2g);
                     float r1, r2;
-                    ca_contra(g, col, is2, rho * s1, rho * s2g, r1, r2);
+                    ca_contra(cm, is2, rho * s1, rho * s2g, r1, r2);
                     de -= n.nu_s * op_wdiv(q, m, r1, r2);
                 }
             }

@@ -175,7 +175,7 @@ struct Loc {
 
 template <bool HYPER, bool VDIFF>
 __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevNum &n, const Nod &q, float W, float iW,
-                                            int col, int k, const float *Y, const float *L) {
+                                            int col, int k, const float *Y, const float *L, const ColMet &cm) {
     const int64_t nC = g.nC, nc = g.ncol;
     const int nl = g.nlev;
     const int64_t idx = (int64_t)k * nc + col;
@@ -192,7 +192,7 @@ __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevN
     c.Jv = g.Jvc[k];
     c.m = ca_met(W, iW, g.Jh[col] * c.s2 * c.Jv, g.iJh[col] * g.isJc[k]);
     const Met &m = c.m;
-    ca_contra(g, col, c.is2, c.u1, c.u2, c.ct1, c.ct2);
+    ca_contra(cm, c.is2, c.u1, c.u2, c.ct1, c.ct2);
     c.Kh = 0.5f * (c.u1 * c.ct1 + c.u2 * c.ct2);
     c.Kv = 0.25f * (c.u3a * g.g33f[k] * c.u3a + c.u3b * g.g33f[k + 1] * c.u3b);
     const float K = c.Kh + c.Kv;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevN
     if (HYPER) {
         const float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
         float v1, v2;
-        ca_contra(g, col, c.is2, X1, X2, v1, v2);
+        ca_contra(cm, c.is2, X1, X2, v1, v2);
         const float dv = op_div(q, m, v1, v2);
         float gd1, gd2;
         op_wgrad(q, m, dv, gd1, gd2);
@@ -243,20 +243,20 @@ __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevN
         float cr1, cr2;
         op_curl12(q, m, X3, cr1, cr2);
         float cc1, cc2;
-        ca_cov(g, col, c.s2, cr1, cr2, cc1, cc2);
+        ca_cov(cm, c.s2, cr1, cr2, cc1, cc2);
         const float cc3 = c.Jv * c.Jv * cr3;
         const float w3 = op_wcurl3(q, m, cc1, cc2);
         float w1, w2;
         op_wcurl12(q, m, cc3, w1, w2);
         float wc1, wc2;
-        ca_cov(g, col, c.s2, w1, w2, wc1, wc2);
+        ca_cov(cm, c.s2, w1, w2, wc1, wc2);
         const float wc3 = c.Jv * c.Jv * w3;
         d1 -= n.nu_v * (n.dd * gd1 - wc1);
         d2 -= n.nu_v * (n.dd * gd2 - wc2);
         c.l4 = n.dd * 0.0f - wc3;
         op_grad(q, Ls, c.s1, c.s2g);
         float r1, r2;
-        ca_contra(g, col, c.is2, c.rho * c.s1, c.rho * c.s2g, r1, r2);
+        ca_contra(cm, c.is2, c.rho * c.s1, c.rho * c.s2g, r1, r2);
         de -= n.nu_s * op_wdiv(q, m, r1, r2);
     }
     const float om3 = op_wcurl3(q, m, c.u1, c.u2);
@@ -303,7 +303,7 @@ __device__ __forceinline__ void lb_zero(LocBar &b) {
 template <bool HYPER, bool VDIFF>
 __device__ __forceinline__ void loc_reverse(const Loc &c, const LocBar &vb, const float *db, const DevGeom &g,
                                             const DevNum &n, const Nod &q, int col, int k, float *ub, float &u3ab,
-                                            float &u3bb, float *Lb) {
+                                            float &u3bb, float *Lb, const ColMet &cm) {
     const Met &m = c.m;
     const int64_t nc = g.ncol;
     const bool node0 = (q.i | q.j) == 0;
@@ -339,19 +339,19 @@ __device__ __forceinline__ void loc_reverse(const Loc &c, const LocBar &vb, cons
         float r1b = 0.0f, r2b = 0.0f;
         T_wdiv(q, m, -n.nu_s * deb, r1b, r2b);
         float rs1b, rs2b;
-        ca_contra(g, col, c.is2, r1b, r2b, rs1b, rs2b);
+        ca_contra(cm, c.is2, r1b, r2b, rs1b, rs2b);
         rhob += c.s1 * rs1b + c.s2g * rs2b;
         Lb[3] = T_grad(q, c.rho * rs1b, c.rho * rs2b);
         const float gd1b = -n.nu_v * n.dd * d1b, gd2b = -n.nu_v * n.dd * d2b;
         float w1b, w2b;
-        ca_cov(g, col, c.s2, n.nu_v * d1b, n.nu_v * d2b, w1b, w2b);
+        ca_cov(cm, c.s2, n.nu_v * d1b, n.nu_v * d2b, w1b, w2b);
         const float w3b = c.Jv * c.Jv * (-vb.l4);
         const float cc3b = T_wcurl12(q, m, w1b, w2b);
         float cc1b = 0.0f, cc2b = 0.0f;
         T_wcurl3(q, m, w3b, cc1b, cc2b);
         const float cr3b = c.Jv * c.Jv * cc3b;
         float cr1b, cr2b;
-        ca_cov(g, col, c.s2, cc1b, cc2b, cr1b, cr2b);
+        ca_cov(cm, c.s2, cc1b, cc2b, cr1b, cr2b);
         Lb[2] = T_curl12(q, m, cr1b, cr2b);
         float X1b = 0.0f, X2b = 0.0f;
         T_curl3(q, m, cr3b, X1b, X2b);
@@ -359,7 +359,7 @@ __device__ __forceinline__ void loc_reverse(const Loc &c, const LocBar &vb, cons
         float v1b = 0.0f, v2b = 0.0f;
         T_div(q, m, dvb, v1b, v2b);
         float xa, xb;
-        ca_contra(g, col, c.is2, v1b, v2b, xa, xb);
+        ca_contra(cm, c.is2, v1b, v2b, xa, xb);
         Lb[0] = X1b + xa;
         Lb[1] = X2b + xb;
     } else {
@@ -423,7 +423,7 @@ __device__ __forceinline__ void loc_reverse(const Loc &c, const LocBar &vb, cons
     u3ab = Kvb * 0.5f * g.g33f[k] * c.u3a;
     u3bb = Kvb * 0.5f * g.g33f[k + 1] * c.u3b;
     float a1, a2;
-    ca_contra(g, col, c.is2, ct1b, ct2b, a1, a2);
+    ca_contra(cm, c.is2, ct1b, ct2b, a1, a2);
     ub[0] = rhob;
     ub[1] = u1b + a1;
     ub[2] = u2b + a2;
@@ -461,6 +461,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
     const float W = c_W[q.j * 4 + q.i], iW = c_iW[q.j * 4 + q.i];
     const float Jh = g.Jh[col], iJh = g.iJh[col];
     const float f12a = g.f12a[col], f12b = g.f12b[col];
+    const ColMet cm = ca_colmet(g, col);
     float Kf = 0.0f;
     bool kfree = false;
     if (VDIFF) {
@@ -482,7 +483,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
     for (int k = kbeg; k <= k1; ++k) {
         // ---- centre k: forward, tendency cotangent
         if (k < nl) {
-            loc_forward<HYPER, VDIFF>(cur, g, n, q, W, iW, col, k, Y, L);
+            loc_forward<HYPER, VDIFF>(cur, g, n, q, W, iW, col, k, Y, L, cm);
             const int64_t idx = (int64_t)k * nc + col;
             cY[0] = Ytb[idx];
             cY[1] = Ytb[nC + idx];
@@ -610,7 +611,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
                 pb.irho += -(c1 * pY[1] + c2 * pY[2]);
             }
             float ubc[4], Lbc[4], u3ab, u3bb;
-            loc_reverse<HYPER, VDIFF>(prev, pb, pY, g, n, q, col, kk, ubc, u3ab, u3bb, Lbc);
+            loc_reverse<HYPER, VDIFF>(prev, pb, pY, g, n, q, col, kk, ubc, u3ab, u3bb, Lbc, cm);
             const int64_t idx = (int64_t)kk * nc + col;
             Yb[idx] = ubc[0];
             Yb[nC + idx] = ubc[1];

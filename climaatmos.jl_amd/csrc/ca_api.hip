@@ -63,6 +63,7 @@ struct ca_handle {
     int newton_variant = 1;
     int explicit_chunk = 16;
     int dss_chunk = 8;
+    int lap_chunk = 4;  // levels per work item of k_laplacians
     int dss_kind = 0;  // 0: LDS-staged per element cluster, 1: direct gather
     // reverse mode (ca_step_adjoint and the *_vjp hooks)
     DevGeom gT{};                 // g with the transposed DSS tables
@@ -233,7 +234,8 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
     if (n.sphere && n.hyper) {
         {
             Launch l(h, st, KID_LAPLACIANS);
-            k_laplacians<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, n, Y, h->raw);
+            k_laplacians<<<dim3(cdiv(g.ncol, 256), cdiv(g.nlev, h->lap_chunk)), 256, 0, st>>>(g, n, Y, h->raw,
+                                                                                       h->lap_chunk);
         }
         {
             Launch l(h, st, KID_DSS_LAPLACIAN);
@@ -554,6 +556,7 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
         const int64_t want = (1024 + layer_waves - 1) / layer_waves;
         h->explicit_chunk = (int)std::min<int64_t>(16, std::max<int64_t>(4, (nl + want - 1) / want));
         h->adj_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
+        h->lap_chunk = (int)std::min<int64_t>(4, std::max<int64_t>(2, (nl + want - 1) / want));
         if (g.ncl > 0) {
             const int64_t wantd = (1024 + g.ncl - 1) / g.ncl;
             h->dss_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(2, (nl + wantd - 1) / wantd));

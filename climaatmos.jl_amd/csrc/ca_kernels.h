@@ -49,7 +49,7 @@ template <int TC, int NF, int NI>
 __global__ void k_newton_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI, float *Ut);
 
 // explicit tendency (ca_spectral.hip)
-__global__ void k_laplacians(DevGeom g, DevNum n, const float *Y, float *L);
+__global__ void k_laplacians(DevGeom g, DevNum n, const float *Y, float *L, int chunk);
 template <bool SPH, bool HYPER, bool VDIFF>
 __global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk);
 

@@ -244,7 +244,8 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
     if (h->n.hyper) {  // the DSSed Laplacians at Y
         {
             Launch l(h, st, KID_LAPLACIANS);
-            k_laplacians<<<dim3(cdiv(g.ncol, 256), g.nlev), 256, 0, st>>>(g, h->n, Y, h->raw);
+            k_laplacians<<<dim3(cdiv(g.ncol, 256), cdiv(g.nlev, h->lap_chunk)), 256, 0, st>>>(g, h->n, Y, h->raw,
+                                                                                       h->lap_chunk);
         }
         {
             Launch l(h, st, KID_DSS_LAPLACIAN);

@@ -265,18 +265,33 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
     return CA_OK;
 }
 
-template <int TC>
-void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st) {
+inline bool ca_al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+template <int TC, bool V4>
+void launch_pcr_tcv(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st) {
     const DevGeom &g = h->g;
     const dim3 grid(cdiv(g.ncol, TC)), blk(256);
     if (src.nfull == 2 && src.nfi == 0)
-        k_newton_pcr<TC, 2, 0><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, 2, 0, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
     else if (src.nfull == 3 && src.nfi == 1)
-        k_newton_pcr<TC, 3, 1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, 3, 1, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
     else if (src.nfull == 4 && src.nfi == 2)
-        k_newton_pcr<TC, 4, 2><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, 4, 2, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
     else
-        k_newton_pcr<TC, -1, -1><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, -1, -1, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+}
+
+// 16-byte row accesses when every row of every plane is 16-byte aligned:
+// ncol a multiple of 4 and all base pointers 16-byte aligned
+template <int TC>
+void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st) {
+    bool v4 = TC == 16 && h->g.ncol % 4 == 0 && ca_al16(U) && ca_al16(h->FI[i]) && (!Ut || ca_al16(Ut));
+    for (int t = 0; t < src.nfull; ++t) v4 = v4 && ca_al16(src.full[t]);
+    for (int t = 0; t < src.nfi; ++t) v4 = v4 && ca_al16(src.fi[t]);
+    if (v4)
+        launch_pcr_tcv<16, true>(h, src, dtg, i, U, Ut, st);
+    else
+        launch_pcr_tcv<TC, false>(h, src, dtg, i, U, Ut, st);
 }
 
 // the implicit stage i from src into U (h->U unless the caller records);

@@ -59,7 +59,52 @@ __device__ __forceinline__ void assemble(const StageSrc &src, int64_t nC, int64_
     out[4] = s;
 }
 
-template <int TC, int NF, int NI>
+// As assemble, for the 4 adjacent columns q .. q + 3 with 16-byte loads
+// (q, nC and the source pointers 16-byte aligned); the same fmaf order per
+// column, so the sums are bitwise those of assemble.
+__device__ __forceinline__ float4 ca_ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void ca_fma4(float a, const float4 &x, float4 &s) {
+    s.x = fmaf(a, x.x, s.x);
+    s.y = fmaf(a, x.y, s.y);
+    s.z = fmaf(a, x.z, s.z);
+    s.w = fmaf(a, x.w, s.w);
+}
+template <int NF, int NI>
+__device__ __forceinline__ void assemble4(const StageSrc &src, int64_t nC, int64_t q, bool okc, bool cen,
+                                          bool face, float4 out[5]) {
+    const int nf = NF >= 0 ? NF : src.nfull;
+    const int ni = NI >= 0 ? NI : src.nfi;
+#pragma unroll
+    for (int comp = 0; comp < 4; ++comp) {
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (okc && cen) {
+#pragma unroll
+            for (int t = 0; t < (NF >= 0 ? NF : CA_MAX_SRC); ++t)
+                if (t < nf) ca_fma4(src.cf[t], ca_ld4(src.full[t] + comp * nC + q), s);
+            if (comp == 0 || comp == 3) {
+                const int ci = comp == 0 ? 0 : 1;
+#pragma unroll
+                for (int t = 0; t < (NI >= 0 ? NI : CA_MAX_SRC); ++t)
+                    if (t < ni) ca_fma4(src.ci[t], ca_ld4(src.fi[t] + ci * nC + q), s);
+            }
+        }
+        out[comp] = s;
+    }
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (okc && face) {
+#pragma unroll
+        for (int t = 0; t < (NF >= 0 ? NF : CA_MAX_SRC); ++t)
+            if (t < nf) ca_fma4(src.cf[t], ca_ld4(src.full[t] + 4 * nC + q), s);
+#pragma unroll
+        for (int t = 0; t < (NI >= 0 ? NI : CA_MAX_SRC); ++t)
+            if (t < ni) ca_fma4(src.ci[t], ca_ld4(src.fi[t] + 2 * nC + q), s);
+    }
+    out[4] = s;
+}
+
+// V4: 16-byte row accesses in phases 1 and 3 (ncol a multiple of 4 and every
+// source and destination 16-byte aligned; the host checks).
+template <int TC, int NF, int NI, bool V4>
 __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U,
                                                     float *FI, float *Ut) {
     constexpr int R = TC + 1;  // padded LDS row (conflict-free column reads)
@@ -71,6 +116,28 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
     const int tid = threadIdx.x;
 
     // ---- phase 1: assemble U~ (coalesced rows of TC columns)
+    if constexpr (V4) {
+        for (int idx = tid; idx < 16 * TC; idx += 256) {
+            const int r = idx / (TC / 4), c4 = 4 * (idx - r * (TC / 4));
+            const int col = col0 + c4;
+            const int64_t q = (int64_t)r * nc + col;
+            float4 v[5];
+            assemble4<NF, NI>(src, nC, q, col < nc, r < nl, r >= 1 && r <= nl - 1, v);
+#pragma unroll
+            for (int comp = 0; comp < 5; ++comp) {
+                tU[comp][r][c4] = v[comp].x;
+                tU[comp][r][c4 + 1] = v[comp].y;
+                tU[comp][r][c4 + 2] = v[comp].z;
+                tU[comp][r][c4 + 3] = v[comp].w;
+            }
+            if (Ut && col < nc) {
+                if (r < nl)
+#pragma unroll
+                    for (int comp = 0; comp < 4; ++comp) *reinterpret_cast<float4 *>(Ut + comp * nC + q) = v[comp];
+                if (r <= nl) *reinterpret_cast<float4 *>(Ut + 4 * nC + q) = v[4];
+            }
+        }
+    } else {
 #pragma unroll 2
     for (int idx = tid; idx < 64 * TC; idx += 256) {
         const int r = idx / TC, cc = idx - r * TC;
@@ -87,6 +154,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
                 for (int comp = 0; comp < 4; ++comp) Ut[comp * nC + q] = v[comp];
             if (r <= nl) Ut[4 * nC + q] = v[4];
         }
+    }
     }
     __syncthreads();
 
@@ -228,6 +296,28 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
     __syncthreads();
 
     // ---- phase 3: coalesced stores of U and F_I
+    if constexpr (V4) {
+        for (int idx = tid; idx < 16 * TC; idx += 256) {
+            const int r = idx / (TC / 4), c4 = 4 * (idx - r * (TC / 4));
+            const int col = col0 + c4;
+            if (col >= nc) continue;
+            const int64_t q = (int64_t)r * nc + col;
+            auto row4 = [&](const float (*t)[R]) {
+                return make_float4(t[r][c4], t[r][c4 + 1], t[r][c4 + 2], t[r][c4 + 3]);
+            };
+            if (r < nl) {
+#pragma unroll
+                for (int comp = 0; comp < 4; ++comp) *reinterpret_cast<float4 *>(U + comp * nC + q) = row4(tU[comp]);
+                *reinterpret_cast<float4 *>(FI + q) = row4(tF[0]);
+                *reinterpret_cast<float4 *>(FI + nC + q) = row4(tF[1]);
+            }
+            if (r <= nl) {
+                *reinterpret_cast<float4 *>(U + 4 * nC + q) = row4(tU[4]);
+                *reinterpret_cast<float4 *>(FI + 2 * nC + q) = row4(tF[2]);
+            }
+        }
+        return;
+    }
     for (int idx = tid; idx < 64 * TC; idx += 256) {
         const int r = idx / TC, cc = idx - r * TC;
         const int col = col0 + cc;
@@ -246,13 +336,18 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
     }
 }
 
-#define CA_INST_PCR(TC, NF, NI) \
-    template __global__ void k_newton_pcr<TC, NF, NI>(DevGeom, DevNum, StageSrc, float, float *, float *, float *);
-CA_INST_PCR(16, 2, 0)
-CA_INST_PCR(16, 3, 1)
-CA_INST_PCR(16, 4, 2)
-CA_INST_PCR(16, -1, -1)
-CA_INST_PCR(32, 2, 0)
-CA_INST_PCR(32, 3, 1)
-CA_INST_PCR(32, 4, 2)
-CA_INST_PCR(32, -1, -1)
+#define CA_INST_PCR(TC, NF, NI, V4)                                                                   \
+    template __global__ void k_newton_pcr<TC, NF, NI, V4>(DevGeom, DevNum, StageSrc, float, float *, float *, \
+                                                          float *);
+CA_INST_PCR(16, 2, 0, false)
+CA_INST_PCR(16, 3, 1, false)
+CA_INST_PCR(16, 4, 2, false)
+CA_INST_PCR(16, -1, -1, false)
+CA_INST_PCR(16, 2, 0, true)
+CA_INST_PCR(16, 3, 1, true)
+CA_INST_PCR(16, 4, 2, true)
+CA_INST_PCR(16, -1, -1, true)
+CA_INST_PCR(32, 2, 0, false)
+CA_INST_PCR(32, 3, 1, false)
+CA_INST_PCR(32, 4, 2, false)
+CA_INST_PCR(32, -1, -1, false)

@@ -195,12 +195,22 @@ inline unsigned dss_lds_grid(const DevGeom &g, int chunk) {
 
 // DSS of NC centre planes (vector pair at VEC) and, with FACE, a face plane,
 // src -> dst: the LDS-staged kernel on the sphere
+// LDS-staged DSS; 16-byte staging loads when src is 16-byte aligned (the
+// planes and rows of a sphere grid then are too: ncol is a multiple of 16)
+template <int NC, int VEC, bool FACE>
+void launch_dss_lds(const DevGeom &g, const float *src, float *dst, int filter, int ch, hipStream_t st) {
+    if (((uintptr_t)src & 15) == 0)
+        k_dss_lds<NC, VEC, FACE, true><<<dss_lds_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+    else
+        k_dss_lds<NC, VEC, FACE, false><<<dss_lds_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+}
+
 template <int NC, int VEC, bool FACE>
 void launch_dss(const ca_handle *h, const float *src, float *dst, int filter, hipStream_t st) {
     const DevGeom &g = h->g;
     const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
     if (h->dss_kind == 0 && g.ncl > 0)
-        k_dss_lds<NC, VEC, FACE><<<dss_lds_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+        launch_dss_lds<NC, VEC, FACE>(g, src, dst, filter, ch, st);
     else
         k_dss_gather<NC, VEC, FACE><<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
 }

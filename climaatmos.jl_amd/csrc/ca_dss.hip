@@ -119,7 +119,9 @@ template __global__ void k_dss_gather<4, 0, false>(DevGeom, const float *, float
 // is staged and used.
 // Sums run over the same canonical slots with the same weights as the
 // direct form: the results are bitwise identical to k_dss_gather's.
-template <int NC, int VEC, bool FACE>
+// V4: the staged element rows are loaded 4 nodes at a time (16-byte loads
+// and LDS writes; src 16-byte aligned, which the host checks).
+template <int NC, int VEC, bool FACE, bool V4>
 __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restrict__ src,
                                                  float *__restrict__ dst, int filter, int chunk) {
     constexpr int NP = NC + (FACE ? 1 : 0);
@@ -145,22 +147,42 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
     const int k1 = min(k0 + chunk, nl);
     const int kf1 = (FACE && k1 == nl) ? nl + 1 : k1;
     // staged values of this thread: entries tid + 256 j, j < 4, of every plane
-    constexpr int NJ = (CA_DSS_EMAX * 16 + 255) / 256;
+    // (V4: the 4-node groups tid + 256 j of every plane)
+    constexpr int NJ = V4 ? (CA_DSS_EMAX * 4 + 255) / 256 : (CA_DSS_EMAX * 16 + 255) / 256;
+    constexpr int W = V4 ? 4 : 1;
     int64_t soff[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         const int i = tid + 256 * j;
-        soff[j] = i < nS ? 16 * (int64_t)list[i >> 4] + (i & 15) : -1;
+        if (V4)
+            soff[j] = i < nS / 4 ? 16 * (int64_t)list[i >> 2] + 4 * (i & 3) : -1;
+        else
+            soff[j] = i < nS ? 16 * (int64_t)list[i >> 4] + (i & 15) : -1;
     }
-    float pre[NJ][NP];
+    float pre[NJ][NP][W];
     auto fetch = [&](int k) {
         const int64_t row = (int64_t)k * nc;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
             if (soff[j] < 0) continue;
 #pragma unroll
-            for (int p = 0; p < NC; ++p) pre[j][p] = k < nl ? src[p * nC + row + soff[j]] : 0.0f;
-            if (FACE) pre[j][NC] = src[NC * nC + row + soff[j]];
+            for (int p = 0; p < NP; ++p) {
+                if (p < NC && k >= nl) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) pre[j][p][w] = 0.0f;
+                    continue;
+                }
+                const float *a = src + p * nC + row + soff[j];
+                if constexpr (V4) {
+                    const float4 v = *reinterpret_cast<const float4 *>(a);
+                    pre[j][p][0] = v.x;
+                    pre[j][p][1] = v.y;
+                    pre[j][p][2] = v.z;
+                    pre[j][p][3] = v.w;
+                } else {
+                    pre[j][p][0] = *a;
+                }
+            }
         }
     };
     // level k + 1 is loaded into registers while level k is staged and used
@@ -171,7 +193,13 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
         for (int j = 0; j < NJ; ++j) {
             if (soff[j] < 0) continue;
 #pragma unroll
-            for (int p = 0; p < NP; ++p) sh[buf][p][tid + 256 * j] = pre[j][p];
+            for (int p = 0; p < NP; ++p) {
+                if constexpr (V4)
+                    *reinterpret_cast<float4 *>(&sh[buf][p][4 * (tid + 256 * j)]) =
+                        make_float4(pre[j][p][0], pre[j][p][1], pre[j][p][2], pre[j][p][3]);
+                else
+                    sh[buf][p][tid + 256 * j] = pre[j][p][0];
+            }
         }
         if (k + 1 < kf1) fetch(k + 1);
         __syncthreads();
@@ -212,5 +240,7 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
     }
 }
 
-template __global__ void k_dss_lds<4, 1, true>(DevGeom, const float *, float *, int, int);
-template __global__ void k_dss_lds<4, 0, false>(DevGeom, const float *, float *, int, int);
+template __global__ void k_dss_lds<4, 1, true, false>(DevGeom, const float *, float *, int, int);
+template __global__ void k_dss_lds<4, 0, false, false>(DevGeom, const float *, float *, int, int);
+template __global__ void k_dss_lds<4, 1, true, true>(DevGeom, const float *, float *, int, int);
+template __global__ void k_dss_lds<4, 0, false, true>(DevGeom, const float *, float *, int, int);

@@ -59,6 +59,6 @@ __global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, 
 template <int NC, int VEC, bool FACE>
 __global__ void k_dss_gather(DevGeom g, const float *src, float *dst, int filter, int chunk);
 // the same, staged through LDS per element cluster
-template <int NC, int VEC, bool FACE>
+template <int NC, int VEC, bool FACE, bool V4>
 __global__ void k_dss_lds(DevGeom g, const float *src, float *dst, int filter, int chunk);
 __global__ void k_combine(DevGeom g, const float *u, float *out, FinalSrc s);

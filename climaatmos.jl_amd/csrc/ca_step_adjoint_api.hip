@@ -59,7 +59,7 @@ ca_status dss_T_state(ca_handle *h, const float *src, float *dst, int filter, hi
     const DevGeom &g = h->gT;
     const int ch = h->dss_chunk > 0 ? h->dss_chunk : g.nlev;
     if (h->dss_kind == 0 && g.ncl > 0)
-        k_dss_lds<4, 1, true><<<dss_lds_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
+        launch_dss_lds<4, 1, true>(g, src, dst, filter, ch, st);
     else
         k_dss_gather<4, 1, true><<<dss_grid(g, ch), 256, 0, st>>>(g, src, dst, filter, ch);
     return check_launch(h, "k_dss(transposed state)");
@@ -131,7 +131,7 @@ ca_status explicit_vjp(ca_handle *h, const float *U, const float *L, const float
         const DevGeom &gT = h->gT;
         const int dch = h->dss_chunk > 0 ? h->dss_chunk : gT.nlev;
         if (h->dss_kind == 0 && gT.ncl > 0)
-            k_dss_lds<4, 0, false><<<dss_lds_grid(gT, dch), 256, 0, st>>>(gT, Lb, Lb2, 0, dch);
+            launch_dss_lds<4, 0, false>(gT, Lb, Lb2, 0, dch, st);
         else
             k_dss_gather<4, 0, false><<<dss_grid(gT, dch), 256, 0, st>>>(gT, Lb, Lb2, 0, dch);
     }

@@ -310,6 +310,58 @@ __device__ __forceinline__ void op_wcurl12(const Nod &q, const Met &m, float u3,
     o2 = ca_d1t(q, Wu) * m.iWJ;
 }
 
+// ----------------------------------------------------------- Laplacians
+// prep_hyperdiffusion_tendency! (hyperdiffusion.jl:141-147, dry) at one point.
+// The four Laplacian planes at one (node, level): lanes 16 r .. 16 r + 15
+// hold the 16 nodes of one element at one level (DPP rows); cm, Jh, iJh:
+// the column's horizontal metric; u3a, u3b: u_3 at the faces below and above.
+// The s_d - s_d,r field is taken relative to its value at the element's first
+// node (a constant inside the element, annihilated by the gradient), which
+// removes the large geopotential part.
+__device__ __forceinline__ void lap_point(const DevGeom &g, const DevNum &n, const Nod &q, float W, float iW,
+                                          float Jh, float iJh, const ColMet &cm, int k, float rho, float u1,
+                                          float u2, float rhoe, float u3a, float u3b, float out[4]) {
+    const float s = g.sc[k], s2 = s * s, is2 = g.is2c[k], Jv = g.Jvc[k];
+    const Met m = ca_met(W, iW, Jh * s2 * Jv, iJh * g.isJc[k]);
+    float ct1, ct2;
+    ca_contra(cm, is2, u1, u2, ct1, ct2);
+    const float ga = g.g33f[k], gb = g.g33f[k + 1];
+    const float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
+    const double e_tot_d = (double)rhoe * ca_drcp((double)rho);
+    const double Td = fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) * n.dicv_d, n.dTmin);
+    const float T = (float)Td;
+    const float p = (float)((double)rho * n.dR_d * Td);
+    const float Pi = ca_exner(n, p);
+    const float cu3 = 0.5f * (u3a + u3b);
+    // vector Laplacian of c_u
+    const float dv = op_div(q, m, ct1, ct2);
+    float gd1, gd2;
+    op_wgrad(q, m, dv, gd1, gd2);
+    const float cr3 = op_curl3(q, m, u1, u2);
+    float cr1, cr2;
+    op_curl12(q, m, cu3, cr1, cr2);
+    float cc1, cc2;
+    ca_cov(cm, s2, cr1, cr2, cc1, cc2);
+    const float cc3 = Jv * Jv * cr3;
+    const float w3 = op_wcurl3(q, m, cc1, cc2);
+    float w1, w2;
+    op_wcurl12(q, m, cc3, w1, w2);
+    float wc1, wc2;
+    ca_cov(cm, s2, w1, w2, wc1, wc2);
+    const float wc3 = Jv * Jv * w3;
+    out[0] = gd1 - wc1;
+    out[1] = gd2 - wc2;
+    out[2] = 0.0f - wc3;
+    // scalar Laplacian of the dry static energy deviation
+    const float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi);
+    const float psi = n.cp_d * (T - ca_Tref(n, Pi)) - ca_dPhiref(n, p0, ca_rcp(p0), Pi0, p);
+    float q1, q2;
+    op_grad(q, psi, q1, q2);
+    float v1, v2;
+    ca_contra(cm, is2, q1, q2, v1, v2);
+    out[3] = op_wdiv(q, m, v1, v2);
+}
+
 // van Leer (Lin et al. 1994) slope with the monotone local-extrema
 // constraint, on the upwind cell a1 with neighbours a0, a2.
 __device__ __forceinline__ float ca_vl_slope(float a0, float a1, float a2) {

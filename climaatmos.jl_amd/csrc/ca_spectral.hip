@@ -26,9 +26,7 @@ __device__ __forceinline__ float u3f_at(const DevGeom &g, const float *u3, int f
 // Work item = (node, chunk of levels): the thread walks its chunk upwards,
 // carrying u_3 of the face between two levels and the column's horizontal
 // metric in registers (each face and metric value is read once); chunk = 0
-// means the whole column.  The s_d - s_d,r field is taken relative to its
-// value at the element's first node (a constant inside the element,
-// annihilated by the gradient), which removes the large geopotential part.
+// means the whole column.
 __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const float *Y, float *L, int chunk) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.ncol) return;
@@ -44,47 +42,11 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
     float u3a = u3f_at(g, u3, k0, col);
     for (int k = k0; k < k1; ++k) {
         const int64_t idx = (int64_t)k * nc + col;
-        const float rho = Y[idx], u1 = Y[nC + idx], u2 = Y[2 * nC + idx], rhoe = Y[3 * nC + idx];
         const float u3b = u3f_at(g, u3, k + 1, col);
-        const float s = g.sc[k], s2 = s * s, is2 = g.is2c[k], Jv = g.Jvc[k];
-        const Met m = ca_met(W, iW, Jh * s2 * Jv, iJh * g.isJc[k]);
-        float ct1, ct2;
-        ca_contra(cm, is2, u1, u2, ct1, ct2);
-        const float ga = g.g33f[k], gb = g.g33f[k + 1];
-        const float K = 0.5f * ((u1 * ct1 + u2 * ct2) + 0.5f * (u3a * ga * u3a + u3b * gb * u3b));
-        const double e_tot_d = (double)rhoe * ca_drcp((double)rho);
-        const double Td = fmax(n.dT_0 + (e_tot_d - (double)K - g.Phic_d[k] + n.dR_d * n.dT_0) * n.dicv_d, n.dTmin);
-        const float T = (float)Td;
-        const float p = (float)((double)rho * n.dR_d * Td);
-        const float Pi = ca_exner(n, p);
-        const float cu3 = 0.5f * (u3a + u3b);
-        // vector Laplacian of c_u
-        const float dv = op_div(q, m, ct1, ct2);
-        float gd1, gd2;
-        op_wgrad(q, m, dv, gd1, gd2);
-        const float cr3 = op_curl3(q, m, u1, u2);
-        float cr1, cr2;
-        op_curl12(q, m, cu3, cr1, cr2);
-        float cc1, cc2;
-        ca_cov(cm, s2, cr1, cr2, cc1, cc2);
-        const float cc3 = Jv * Jv * cr3;
-        const float w3 = op_wcurl3(q, m, cc1, cc2);
-        float w1, w2;
-        op_wcurl12(q, m, cc3, w1, w2);
-        float wc1, wc2;
-        ca_cov(cm, s2, w1, w2, wc1, wc2);
-        const float wc3 = Jv * Jv * w3;
-        L[idx] = gd1 - wc1;
-        L[nC + idx] = gd2 - wc2;
-        L[2 * nC + idx] = 0.0f - wc3;
-        // scalar Laplacian of the dry static energy deviation
-        const float p0 = ca_elem0(p), Pi0 = ca_elem0(Pi);
-        const float psi = n.cp_d * (T - ca_Tref(n, Pi)) - ca_dPhiref(n, p0, ca_rcp(p0), Pi0, p);
-        float q1, q2;
-        op_grad(q, psi, q1, q2);
-        float v1, v2;
-        ca_contra(cm, is2, q1, q2, v1, v2);
-        L[3 * nC + idx] = op_wdiv(q, m, v1, v2);
+        float o[4];
+        lap_point(g, n, q, W, iW, Jh, iJh, cm, k, Y[idx], Y[nC + idx], Y[2 * nC + idx], Y[3 * nC + idx], u3a, u3b, o);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) L[p * nC + idx] = o[p];
         u3a = u3b;
     }
 }

@@ -6,10 +6,18 @@
 namespace {
 
 // Element clusters of the LDS-staged DSS.  Elements are adjacent if they
-// share a DSS node.  Clusters grow breadth-first over edge neighbours
-// (elements sharing a node of multiplicity 2) from the lowest unassigned
-// element, up to CA_DSS_OWN elements while own + neighbouring elements stay
-// within CA_DSS_EMAX.  Every index the kernel uses is checked here.
+// share a DSS node.  A cluster stages its own elements and all their
+// neighbours, so compact clusters read fewer halo rows.
+//   * Panel blocks: when the elements are numbered panel by panel and row by
+//     row on 6 panels of h x h (e = p h^2 + y h + x; checked against the edge
+//     adjacency: e + 1 and e + h are edge neighbours inside a panel), the
+//     clusters are the 4 x 4 blocks of each panel (the last row and column of
+//     blocks narrower when 4 does not divide h).
+//   * Otherwise clusters grow breadth-first over edge neighbours (elements
+//     sharing a node of multiplicity 2) from the lowest unassigned element,
+//     up to CA_DSS_OWN elements while own + neighbouring elements stay within
+//     CA_DSS_EMAX.
+// Every index the kernel uses is checked here.
 std::string build_clusters(int ne, DssTable &T) {
     const int nc = 16 * ne;
     std::vector<std::vector<int>> adj(ne), eadj(ne);
@@ -39,20 +47,54 @@ std::string build_clusters(int ne, DssTable &T) {
         return u;
     };
     std::vector<int> owner(ne, -1), pos(ne, -1);
-    for (int e0 = 0; e0 < ne; ++e0) {
-        if (owner[e0] >= 0) continue;
+    // panel-block numbering check
+    int hp = 0;
+    while (6 * (hp + 1) * (hp + 1) <= ne) ++hp;
+    bool panels = hp >= 1 && 6 * hp * hp == ne;
+    auto is_edge = [&](int a, int b) { return std::binary_search(eadj[a].begin(), eadj[a].end(), b); };
+    for (int e = 0; panels && e < ne; ++e) {
+        const int x = e % hp, y = (e / hp) % hp;
+        if ((x + 1 < hp && !is_edge(e, e + 1)) || (y + 1 < hp && !is_edge(e, e + hp))) panels = false;
+    }
+    std::vector<std::vector<int>> blocks;
+    if (panels) {
+        constexpr int B = 4;  // 4 x 4 = CA_DSS_OWN elements
+        for (int p = 0; p < 6; ++p)
+            for (int by = 0; by < hp; by += B)
+                for (int bx = 0; bx < hp; bx += B) {
+                    std::vector<int> own;
+                    for (int y = by; y < std::min(by + B, hp); ++y)
+                        for (int x = bx; x < std::min(bx + B, hp); ++x) own.push_back(p * hp * hp + y * hp + x);
+                    if ((int)staged(own).size() > CA_DSS_EMAX) {
+                        panels = false;
+                        break;
+                    }
+                    blocks.push_back(own);
+                }
+        if (!panels) blocks.clear();
+    }
+    static_assert(CA_DSS_OWN == 16, "panel blocks are 4 x 4 elements");
+    for (int e0 = 0, bi = 0; panels ? bi < (int)blocks.size() : e0 < ne; panels ? ++bi : ++e0) {
+        if (!panels && owner[e0] >= 0) continue;
         const int cid = (int)T.nE.size();
-        std::vector<int> own{e0}, queue(eadj[e0].begin(), eadj[e0].end());
-        owner[e0] = cid;
-        for (size_t qi = 0; qi < queue.size() && (int)own.size() < CA_DSS_OWN; ++qi) {
-            const int cand = queue[qi];
-            if (owner[cand] >= 0) continue;
-            std::vector<int> trial(own);
-            trial.push_back(cand);
-            if ((int)staged(trial).size() > CA_DSS_EMAX) continue;
-            own.push_back(cand);
-            owner[cand] = cid;
-            queue.insert(queue.end(), eadj[cand].begin(), eadj[cand].end());
+        std::vector<int> own;
+        if (panels) {
+            own = blocks[bi];
+            for (int e : own) owner[e] = cid;
+        } else {
+            own.push_back(e0);
+            std::vector<int> queue(eadj[e0].begin(), eadj[e0].end());
+            owner[e0] = cid;
+            for (size_t qi = 0; qi < queue.size() && (int)own.size() < CA_DSS_OWN; ++qi) {
+                const int cand = queue[qi];
+                if (owner[cand] >= 0) continue;
+                std::vector<int> trial(own);
+                trial.push_back(cand);
+                if ((int)staged(trial).size() > CA_DSS_EMAX) continue;
+                own.push_back(cand);
+                owner[cand] = cid;
+                queue.insert(queue.end(), eadj[cand].begin(), eadj[cand].end());
+            }
         }
         // staged list: own elements, then the other staged elements ascending
         std::vector<int> lst(own);

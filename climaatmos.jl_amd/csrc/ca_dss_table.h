@@ -8,7 +8,7 @@
 #include <vector>
 
 #define CA_DSS_OWN 16   // own elements per DSS cluster (256 columns, one workgroup)
-#define CA_DSS_EMAX 64  // staged elements per cluster, own + neighbours
+#define CA_DSS_EMAX 48  // staged elements per cluster, own + neighbours
 
 struct DssTable {
     // per column, 4 slots: the copies of the column's node in the group's

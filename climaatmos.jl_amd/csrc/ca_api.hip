@@ -61,7 +61,7 @@ struct ca_handle {
     std::vector<ProfRec> recs;
     double aE[4][4], aI[4][4], b[4];
     int newton_variant = 1;
-    int explicit_chunk = 12;
+    int explicit_chunk = 8;
     int dss_chunk = 8;
     int lap_chunk = 4;  // levels per work item of k_laplacians
     int dss_kind = 0;  // 0: LDS-staged per element cluster, 1: direct gather
@@ -579,7 +579,7 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     {
         const int64_t layer_waves = (nc + 63) / 64;
         const int64_t want = (1024 + layer_waves - 1) / layer_waves;
-        h->explicit_chunk = (int)std::min<int64_t>(12, std::max<int64_t>(4, (nl + want - 1) / want));
+        h->explicit_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
         h->adj_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
         h->lap_chunk = (int)std::min<int64_t>(4, std::max<int64_t>(2, (nl + want - 1) / want));
         if (g.ncl > 0) {

@@ -118,7 +118,12 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             float p = (float)((double)rho * n.dR_d * Td);
             float Pi = ca_exner(n, p);
             float dr = 0.0f, d1 = 0.0f, d2 = 0.0f, de = 0.0f, l4 = 0.0f;
-            if (sph) {
+            // The chunk's halo centres (k0 - 1 below, k1 above; uniform per
+            // workgroup) only feed the face stencils of faces k0 and k1: they
+            // need K, the velocities, rho J, l4 and the diffusion inputs, not
+            // their own tendencies.
+            const bool own = k >= k0 && k < k1;
+            if (sph && own) {
                 // ---- horizontal_dynamics_tendency! (advection.jl:36-91)
                 float m1 = rho * ct1, m2 = rho * ct2;
                 float wd = op_wdiv(q, m, m1, m2);
@@ -145,6 +150,18 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 op_grad(q, th, c1, c2);
                 d1 -= g1 + n.cp_d * (th * a1 + b1 - Pi * c1) * 0.5f;
                 d2 -= g2 + n.cp_d * (th * a2 + b2 - Pi * c2) * 0.5f;
+            }
+            if (sph && HYPER && !own) {  // l4 alone, as below
+                const float X3 = L[2 * nC + idx];
+                float cr1, cr2;
+                op_curl12(q, m, X3, cr1, cr2);
+                float cc1, cc2;
+                ca_cov(cm, s2, cr1, cr2, cc1, cc2);
+                const float w3 = op_wcurl3(q, m, cc1, cc2);
+                const float wc3 = Jv * Jv * w3;
+                l4 = n.dd * 0.0f - wc3;
+            }
+            if (sph && own) {
                 // ---- apply_hyperdiffusion_tendency! (hyperdiffusion.jl:273-307)
                 if (HYPER) {
                     float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
@@ -175,16 +192,18 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     de -= n.nu_s * op_wdiv(q, m, r1, r2);
                 }
             }
-            // ---- explicit_vertical_advection_tendency! centre part (advection.jl:273-277)
-            float om3 = sph ? op_wcurl3(q, m, u1, u2) : 0.0f;
-            float a3 = f3n * g.iJvc[k] + om3;
-            d1 -= J * (0.0f - a3 * ct2);
-            d2 -= J * (a3 * ct1 - 0.0f);
-            // ---- Rayleigh sponge on u_h (rayleigh_sponge.jl:39-44)
-            if (n.rayleigh && n.alpha_uh != 0.0f) {
-                float beta = n.alpha_uh * g.zeta_c[k];
-                d1 += -beta * u1;
-                d2 += -beta * u2;
+            if (own) {
+                // ---- explicit_vertical_advection_tendency! centre part (advection.jl:273-277)
+                float om3 = sph ? op_wcurl3(q, m, u1, u2) : 0.0f;
+                float a3 = f3n * g.iJvc[k] + om3;
+                d1 -= J * (0.0f - a3 * ct2);
+                d2 -= J * (a3 * ct1 - 0.0f);
+                // ---- Rayleigh sponge on u_h (rayleigh_sponge.jl:39-44)
+                if (n.rayleigh && n.alpha_uh != 0.0f) {
+                    float beta = n.alpha_uh * g.zeta_c[k];
+                    d1 += -beta * u1;
+                    d2 += -beta * u2;
+                }
             }
             cur.rhoJ = rho * J;
             cur.irhoJ = ca_rcp(cur.rhoJ);

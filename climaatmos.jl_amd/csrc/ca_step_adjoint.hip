@@ -766,6 +766,55 @@ __global__ void __launch_bounds__(256) k_lincomb(DevGeom g, float *out, Lin L) {
     }
 }
 
+// As k_lincomb, 4 consecutive entries per thread with 16-byte accesses (every
+// plane a multiple of 4 entries and every pointer 16-byte aligned: the host
+// checks); the same arithmetic per entry.
+__device__ __forceinline__ float4 lin_ld4(const float *p, int64_t i) { return *reinterpret_cast<const float4 *>(p + i); }
+__global__ void __launch_bounds__(256) k_lincomb4(DevGeom g, float *out, Lin L) {
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const int64_t S = 4 * g.nC + g.nF;
+    if (i >= S) return;
+    const int p = i < 4 * g.nC ? (int)(i / g.nC) : 4;
+    float4 x[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) x[t] = t < L.nx ? lin_ld4(L.x[t], i) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if ((L.planes >> p) & 1u) {
+        float4 v = L.acc ? lin_ld4(out, i) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            if (t >= L.nx) break;
+            v.x = fmaf(L.a[t], x[t].x, v.x);
+            v.y = fmaf(L.a[t], x[t].y, v.y);
+            v.z = fmaf(L.a[t], x[t].z, v.z);
+            v.w = fmaf(L.a[t], x[t].w, v.w);
+        }
+        if (p == 4 && L.fa) {
+            const int64_t q = i - 4 * g.nC;
+            const int64_t f = q / g.ncol;  // the 4 entries share the face row (ncol % 4 == 0)
+            if (f >= 1 && f <= g.nlev - 1) {
+                const float4 a = lin_ld4(L.fa, q), b = lin_ld4(L.fb, q - g.ncol);
+                v.x += a.x + b.x;
+                v.y += a.y + b.y;
+                v.z += a.z + b.z;
+                v.w += a.w + b.w;
+            }
+        }
+        *reinterpret_cast<float4 *>(out + i) = v;
+    }
+    if (L.out2 && ((L.planes2 >> p) & 1u)) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            if (t >= L.nx) break;
+            v.x = fmaf(L.a2[t], x[t].x, v.x);
+            v.y = fmaf(L.a2[t], x[t].y, v.y);
+            v.z = fmaf(L.a2[t], x[t].z, v.z);
+            v.w = fmaf(L.a2[t], x[t].w, v.w);
+        }
+        *reinterpret_cast<float4 *>(L.out2 + i) = v;
+    }
+}
+
 // u_3 cotangent of the interior faces += the Laplacian reverse's face terms
 __global__ void __launch_bounds__(256) k_fold_faces(DevGeom g, float *Yb, const float *fa, const float *fb) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;

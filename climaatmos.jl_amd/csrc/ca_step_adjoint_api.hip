@@ -91,7 +91,13 @@ ca_status lincomb(ca_handle *h, float *out, std::initializer_list<LinTerm> terms
     L.planes2 = planes2;
     L.fa = fa;
     L.fb = fb;
-    sadj::k_lincomb<<<cdiv(h->S, 256), 256, 0, st>>>(h->g, out, L);
+    auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+    bool v4 = h->g.ncol % 4 == 0 && al(out) && (!out2 || al(out2)) && (!fa || (al(fa) && al(fb)));
+    for (int t = 0; t < L.nx; ++t) v4 = v4 && al(L.x[t]);
+    if (v4)
+        sadj::k_lincomb4<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(h->g, out, L);
+    else
+        sadj::k_lincomb<<<cdiv(h->S, 256), 256, 0, st>>>(h->g, out, L);
     return check_launch(h, "k_lincomb");
 }
 

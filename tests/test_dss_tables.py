@@ -4,6 +4,7 @@ the LDS-staged kernel indexes with.  Every index a kernel uses is checked
 against the geometry's shared-node groups."""
 import ctypes as C
 import os
+import types
 
 import numpy as np
 import pytest
@@ -95,6 +96,55 @@ def test_gather_rows_and_clusters_match_the_groups(lib, h_elem):
         np.testing.assert_array_equal(16 * L[s // 16] + s % 16, tc[cols[act]])
         np.testing.assert_array_equal(T["cw"].reshape(ncl, 256, 4)[cl][act], tw[cols[act]])
     print(f"h{h_elem}: {ncl} clusters, staged elements per own element {nE.sum() / g.nelem:.2f}")
+
+
+@pytest.mark.parametrize("h_elem", [8, 30])
+def test_clusters_are_panel_blocks(lib, h_elem):
+    """On panels numbered row by row the clusters are the 4 x 4 element blocks
+    of each panel (narrower in the last block row / column), each staging at
+    most its 6 x 6 neighbourhood."""
+    g = G.cubed_sphere(h_elem, G.uniform_faces(4, 30000.0), PAR.planet_radius, PAR.Omega)
+    T = build(lib, g)
+    ccol, nE = T["ccol"].reshape(T["ncl"], 256), T["nE"]
+    got = sorted(tuple(sorted(set((c[c >= 0] // 16).tolist()))) for c in ccol)
+    want = []
+    h = h_elem
+    for p in range(6):
+        for by in range(0, h, 4):
+            for bx in range(0, h, 4):
+                want.append(tuple(sorted(p * h * h + y * h + x for y in range(by, min(by + 4, h))
+                                         for x in range(bx, min(bx + 4, h)))))
+    assert got == sorted(want)
+    assert nE.max() <= 36
+
+
+def test_permuted_numbering_falls_back_to_grown_clusters(lib):
+    """A numbering that is not panel by panel, row by row (elements shuffled)
+    still gives valid clusters: a partition of the columns whose slots point
+    at the same copies as the gather rows."""
+    g0 = G.cubed_sphere(8, G.uniform_faces(4, 30000.0), PAR.planet_radius, PAR.Omega)
+    rng = np.random.default_rng(8)
+    perm = rng.permutation(g0.nelem)  # element e becomes perm[e]
+    newcol = (16 * perm[np.arange(g0.ncol) // 16] + np.arange(g0.ncol) % 16).astype(np.int64)
+    basis = np.empty_like(g0.basis)
+    basis[:, newcol] = g0.basis
+    g = types.SimpleNamespace(nelem=g0.nelem, ncol=g0.ncol, dss_ptr=g0.dss_ptr, dss_col=newcol[g0.dss_col],
+                              dss_w=g0.dss_w, basis=basis)
+    T = build(lib, g)
+    nc, ncl, emax = g.ncol, T["ncl"], T["emax"]
+    tc = T["tc"].reshape(nc, 4)
+    ccol, slot = T["ccol"].reshape(ncl, 256), T["slot"].reshape(ncl, 256, 4).astype(np.int64)
+    lst, nE = T["list"].reshape(ncl, emax), T["nE"]
+    owned = ccol[ccol >= 0]
+    assert len(owned) == nc and len(np.unique(owned)) == nc
+    assert np.all(nE <= emax)
+    for cl in range(ncl):
+        cols = ccol[cl]
+        act = cols >= 0
+        s = slot[cl][act]
+        np.testing.assert_array_equal(16 * lst[cl, :nE[cl]][s // 16] + s % 16, tc[cols[act]])
+    # not the panel blocks of the original numbering
+    assert ncl != 6 * 4
 
 
 def test_independent_columns_have_no_clusters(lib):

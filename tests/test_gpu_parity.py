@@ -316,6 +316,27 @@ def test_dss_forms_bitwise_identical_h30():
     assert torch.equal(out[0][1], out[1][1])
 
 
+def test_unaligned_state_steps_bitwise_like_aligned():
+    """A state 4 bytes off 16-byte alignment takes the Newton stage's 4-byte
+    assembly (the 16-byte row loads need every plane aligned); both forms
+    assemble the same sums in the same order, so two steps agree bitwise."""
+    torch = _torch()
+    from cadycore.lib import DryDycore
+
+    g, num, Y0 = _sphere_small()
+    d = DryDycore(g, PAR, num)
+    Ya = d.state(Y0)
+    buf = torch.zeros(d.S + 1, dtype=torch.float32, device=Ya.device)
+    Yu = buf[1:]
+    Yu.copy_(Ya)
+    assert Ya.data_ptr() % 16 == 0 and Yu.data_ptr() % 16 != 0
+    d.step(Ya, 100.0, 2)
+    d.step(Yu, 100.0, 2)
+    torch.cuda.synchronize()
+    d.close()
+    assert torch.equal(Ya.cpu(), Yu.cpu())
+
+
 def test_pack_unpack_roundtrip_climacore_layout():
     """VIJFH parent arrays (v fastest, then i, j, field, element) in and out."""
     torch = _torch()

@@ -337,6 +337,20 @@ def test_unaligned_state_steps_bitwise_like_aligned():
     assert torch.equal(Ya.cpu(), Yu.cpu())
 
 
+def test_create_rejects_a_non_dry_exner_exponent():
+    """The float64 Exner function refines q^(2/7): R_d / cp_d off 2/7 by more
+    than 1e-6 is rejected at create (the parameter set's kappa_d is within 4e-12)."""
+    _torch()
+    import dataclasses
+
+    from cadycore.lib import DryDycore
+
+    g, num, _ = _sphere_small()
+    bad = dataclasses.replace(PAR, cp_d=PAR.R_d / 0.2855)
+    with pytest.raises(RuntimeError, match="2/7"):
+        DryDycore(g, bad, num)
+
+
 def test_pack_unpack_roundtrip_climacore_layout():
     """VIJFH parent arrays (v fastest, then i, j, field, element) in and out."""
     torch = _torch()

@@ -160,6 +160,16 @@ __device__ __forceinline__ void ca_cov(const DevGeom &g, int col, float s2, floa
     o2 = s2 * (g12 * a1 + g22 * a2);
 }
 
+// Access `off` bytes past a wavefront-uniform base pointer.  With the lane's
+// part a 32-bit byte offset the address is a scalar base plus a vector
+// offset: no per-lane 64-bit address arithmetic per access.
+__device__ __forceinline__ float ca_ldo(const float *base, uint32_t off) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void ca_sto(float *base, uint32_t off, float v) {
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 // The horizontal metric of one column (level independent), loaded once by
 // kernels that loop over levels, so the per-level arithmetic reads registers.
 struct ColMet {

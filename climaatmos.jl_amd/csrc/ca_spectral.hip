@@ -80,6 +80,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
     const float Jh = g.Jh[col], iJh = g.iJh[col];
     const float f3n = g.f3n[col], f12a = g.f12a[col], f12b = g.f12b[col];
     const ColMet cm = ca_colmet(g, col);
+    const uint32_t cb = (uint32_t)col * 4u;  // the lane's byte offset in every level row
     float Kf = 0.0f;  // face diffusivity: harmonic mean of the (equal) centre values
     if (VDIFF) {
         const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
@@ -96,10 +97,11 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
     const int kend = k1;  // inclusive: face k1 closes the top owned centre
     float u3a = u3f_at(g, u3, kbeg, col);  // u_3 at face k
     for (int k = kbeg; k <= kend; ++k) {
-        float u3b = (k < nl) ? u3f_at(g, u3, k + 1, col) : 0.0f;
+        float u3b = (k + 1 >= 1 && k + 1 <= nl - 1) ? ca_ldo(u3 + (int64_t)(k + 1) * nc, cb) : 0.0f;
         if (k < nl) {
-            const int64_t idx = (int64_t)k * nc + col;
-            float rho = Y[idx], u1 = Y[nC + idx], u2 = Y[2 * nC + idx], rhoe = Y[3 * nC + idx];
+            const float *Yk = Y + (int64_t)k * nc;
+            float rho = ca_ldo(Yk, cb), u1 = ca_ldo(Yk + nC, cb), u2 = ca_ldo(Yk + 2 * nC, cb),
+                  rhoe = ca_ldo(Yk + 3 * nC, cb);
             float s = g.sc[k], s2 = s * s, is2 = g.is2c[k], Jv = g.Jvc[k];
             const Met m = ca_met(W, iW, Jh * s2 * Jv, iJh * g.isJc[k]);
             const float J = m.J;
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 d2 -= g2 + n.cp_d * (th * a2 + b2 - Pi * c2) * 0.5f;
             }
             if (sph && HYPER && !own) {  // l4 alone, as below
-                const float X3 = L[2 * nC + idx];
+                const float X3 = ca_ldo(L + 2 * nC + (int64_t)k * nc, cb);
                 float cr1, cr2;
                 op_curl12(q, m, X3, cr1, cr2);
                 float cc1, cc2;
@@ -164,7 +166,9 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
             if (sph && own) {
                 // ---- apply_hyperdiffusion_tendency! (hyperdiffusion.jl:273-307)
                 if (HYPER) {
-                    float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
+                    const float *Lk = L + (int64_t)k * nc;
+                    float X1 = ca_ldo(Lk, cb), X2 = ca_ldo(Lk + nC, cb), X3 = ca_ldo(Lk + 2 * nC, cb),
+                          Ls = ca_ldo(Lk + 3 * nC, cb);
                     float v1, v2;
                     ca_contra(cm, is2, X1, X2, v1, v2);
                     float dv = op_div(q, m, v1, v2);
@@ -274,12 +278,11 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                     Fu2 = fac * (cur.uc2 - prev.uc2) * iJv;
                 }
             }
-            if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) Yt[4 * nC + (int64_t)k * nc + col] = du3;
+            if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) ca_sto(Yt + 4 * nC + (int64_t)k * nc, cb, du3);
         }
         // ---- centre k-1 is complete
         if (k - 1 >= k0 && k - 1 < k1) {
             const int kk = k - 1;
-            const int64_t idx = (int64_t)kk * nc + col;
             float d1 = prev.d1 - (0.5f * (Gp1 + G1)) * prev.irhoJ;
             float d2 = prev.d2 - (0.5f * (Gp2 + G2)) * prev.irhoJ;
             float de = prev.de;
@@ -301,10 +304,11 @@ __global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const flo
                 d1 -= c1 * irho;
                 d2 -= c2 * irho;
             }
-            Yt[idx] = prev.dr;
-            Yt[nC + idx] = d1;
-            Yt[2 * nC + idx] = d2;
-            Yt[3 * nC + idx] = de;
+            float *Ytk = Yt + (int64_t)kk * nc;
+            ca_sto(Ytk, cb, prev.dr);
+            ca_sto(Ytk + nC, cb, d1);
+            ca_sto(Ytk + 2 * nC, cb, d2);
+            ca_sto(Ytk + 3 * nC, cb, de);
         }
         Gp1 = G1;
         Gp2 = G2;

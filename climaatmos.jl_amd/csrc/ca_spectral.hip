@@ -39,14 +39,17 @@ __global__ void __launch_bounds__(256) k_laplacians(DevGeom g, DevNum n, const f
     const float W = c_W[q.j * 4 + q.i], iW = c_iW[q.j * 4 + q.i];
     const float Jh = g.Jh[col], iJh = g.iJh[col];
     const ColMet cm = ca_colmet(g, col);
+    const uint32_t cb = (uint32_t)col * 4u;  // the lane's byte offset in every level row
     float u3a = u3f_at(g, u3, k0, col);
     for (int k = k0; k < k1; ++k) {
-        const int64_t idx = (int64_t)k * nc + col;
-        const float u3b = u3f_at(g, u3, k + 1, col);
+        const float *Yk = Y + (int64_t)k * nc;
+        const float u3b = k + 1 <= nl - 1 ? ca_ldo(u3 + (int64_t)(k + 1) * nc, cb) : 0.0f;
         float o[4];
-        lap_point(g, n, q, W, iW, Jh, iJh, cm, k, Y[idx], Y[nC + idx], Y[2 * nC + idx], Y[3 * nC + idx], u3a, u3b, o);
+        lap_point(g, n, q, W, iW, Jh, iJh, cm, k, ca_ldo(Yk, cb), ca_ldo(Yk + nC, cb), ca_ldo(Yk + 2 * nC, cb),
+                  ca_ldo(Yk + 3 * nC, cb), u3a, u3b, o);
+        float *Lk = L + (int64_t)k * nc;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) L[p * nC + idx] = o[p];
+        for (int p = 0; p < 4; ++p) ca_sto(Lk + p * nC, cb, o[p]);
         u3a = u3b;
     }
 }

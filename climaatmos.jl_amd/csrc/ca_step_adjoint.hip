@@ -178,14 +178,14 @@ __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevN
                                             int col, int k, const float *Y, const float *L, const ColMet &cm) {
     const int64_t nC = g.nC, nc = g.ncol;
     const int nl = g.nlev;
-    const int64_t idx = (int64_t)k * nc + col;
-    const float *u3 = Y + 4 * nC;
-    c.rho = Y[idx];
-    c.u1 = Y[nC + idx];
-    c.u2 = Y[2 * nC + idx];
-    c.rhoe = Y[3 * nC + idx];
-    c.u3a = k <= 0 ? 0.0f : u3[idx];
-    c.u3b = k + 1 >= nl ? 0.0f : u3[idx + nc];
+    const uint32_t cb = (uint32_t)col * 4u;  // the lane's byte offset in every level row
+    const float *Yk = Y + (int64_t)k * nc;
+    c.rho = ca_ldo(Yk, cb);
+    c.u1 = ca_ldo(Yk + nC, cb);
+    c.u2 = ca_ldo(Yk + 2 * nC, cb);
+    c.rhoe = ca_ldo(Yk + 3 * nC, cb);
+    c.u3a = k <= 0 ? 0.0f : ca_ldo(Yk + 4 * nC, cb);
+    c.u3b = k + 1 >= nl ? 0.0f : ca_ldo(Yk + 4 * nC + nc, cb);
     c.s = g.sc[k];
     c.s2 = c.s * c.s;
     c.is2 = g.is2c[k];
@@ -233,7 +233,9 @@ __device__ __forceinline__ void loc_forward(Loc &c, const DevGeom &g, const DevN
     c.l4 = 0.0f;
     c.s1 = c.s2g = 0.0f;
     if (HYPER) {
-        const float X1 = L[idx], X2 = L[nC + idx], X3 = L[2 * nC + idx], Ls = L[3 * nC + idx];
+        const float *Lk = L + (int64_t)k * nc;
+        const float X1 = ca_ldo(Lk, cb), X2 = ca_ldo(Lk + nC, cb), X3 = ca_ldo(Lk + 2 * nC, cb),
+                    Ls = ca_ldo(Lk + 3 * nC, cb);
         float v1, v2;
         ca_contra(cm, c.is2, X1, X2, v1, v2);
         const float dv = op_div(q, m, v1, v2);
@@ -462,6 +464,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
     const float Jh = g.Jh[col], iJh = g.iJh[col];
     const float f12a = g.f12a[col], f12b = g.f12b[col];
     const ColMet cm = ca_colmet(g, col);
+    const uint32_t cbo = (uint32_t)col * 4u;  // the lane's byte offset in every level row
     float Kf = 0.0f;
     bool kfree = false;
     if (VDIFF) {
@@ -484,11 +487,11 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
         // ---- centre k: forward, tendency cotangent
         if (k < nl) {
             loc_forward<HYPER, VDIFF>(cur, g, n, q, W, iW, col, k, Y, L, cm);
-            const int64_t idx = (int64_t)k * nc + col;
-            cY[0] = Ytb[idx];
-            cY[1] = Ytb[nC + idx];
-            cY[2] = Ytb[2 * nC + idx];
-            cY[3] = Ytb[3 * nC + idx];
+            const float *Tk = Ytb + (int64_t)k * nc;
+            cY[0] = ca_ldo(Tk, cbo);
+            cY[1] = ca_ldo(Tk + nC, cbo);
+            cY[2] = ca_ldo(Tk + 2 * nC, cbo);
+            cY[3] = ca_ldo(Tk + 3 * nC, cbo);
         }
         lb_zero(cb);
         // ---- face k: forward and reverse (interior faces, prev = k - 1 valid)
@@ -499,7 +502,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
             const float sf = g.sf[k];
             const Met mf = ca_met(W, iW, Jh * sf * sf * g.Jvf[k], iJh * g.isJf[k]);
             const float Jf = mf.J, iJf = mf.iJ;
-            const float u3f = (k >= 1 && k <= nl - 1) ? u3[(int64_t)k * nc + col] : 0.0f;
+            const float u3f = (k >= 1 && k <= nl - 1) ? ca_ldo(u3 + (int64_t)k * nc, cbo) : 0.0f;
             float x = 0.0f, y = 0.0f;
             if (interior) op_wcurl12(q, mf, u3f, x, y);
             if (interior) {
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
                     fk.Fu2 = fac * (cur.uc2 - prev.uc2) * iJv;
                 }
                 // cotangents of this face's outputs, from the completions of centres k - 1 and k
-                const float du3b = Ytb[4 * nC + (int64_t)k * nc + col];
+                const float du3b = ca_ldo(Ytb + 4 * nC + (int64_t)k * nc, cbo);
                 const float G1b = -0.5f * (prev.irhoJ * pY[1] + cur.irhoJ * cY[1]);
                 const float G2b = -0.5f * (prev.irhoJ * pY[2] + cur.irhoJ * cY[2]);
                 // du3 = -(Jf (A1 i2 - A2 i1) + dKh + dKv) - nu_v <l4>_rhoJ
@@ -596,7 +599,7 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
                 }
             }
         }
-        if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) Yb[4 * nC + (int64_t)k * nc + col] = u3face;
+        if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) ca_sto(Yb + 4 * nC + (int64_t)k * nc, cbo, u3face);
         // ---- owned centre k - 1 complete: completion terms that need both of its faces, then its local reverse
         if (k - 1 >= k0 && k > kbeg) {
             const int kk = k - 1;
@@ -612,17 +615,17 @@ __global__ void __launch_bounds__(256) k_explicit_vjp(DevGeom g, DevNum n, const
             }
             float ubc[4], Lbc[4], u3ab, u3bb;
             loc_reverse<HYPER, VDIFF>(prev, pb, pY, g, n, q, col, kk, ubc, u3ab, u3bb, Lbc, cm);
-            const int64_t idx = (int64_t)kk * nc + col;
-            Yb[idx] = ubc[0];
-            Yb[nC + idx] = ubc[1];
-            Yb[2 * nC + idx] = ubc[2];
-            Yb[3 * nC + idx] = ubc[3];
-            Lb[idx] = Lbc[0];
-            Lb[nC + idx] = Lbc[1];
-            Lb[2 * nC + idx] = Lbc[2];
-            Lb[3 * nC + idx] = Lbc[3];
-            fa[idx] = kk >= 1 ? u3ab : 0.0f;
-            fb[idx] = kk + 1 <= nl - 1 ? u3bb : 0.0f;
+            const int64_t row = (int64_t)kk * nc;
+            ca_sto(Yb + row, cbo, ubc[0]);
+            ca_sto(Yb + nC + row, cbo, ubc[1]);
+            ca_sto(Yb + 2 * nC + row, cbo, ubc[2]);
+            ca_sto(Yb + 3 * nC + row, cbo, ubc[3]);
+            ca_sto(Lb + row, cbo, Lbc[0]);
+            ca_sto(Lb + nC + row, cbo, Lbc[1]);
+            ca_sto(Lb + 2 * nC + row, cbo, Lbc[2]);
+            ca_sto(Lb + 3 * nC + row, cbo, Lbc[3]);
+            ca_sto(fa + row, cbo, kk >= 1 ? u3ab : 0.0f);
+            ca_sto(fb + row, cbo, kk + 1 <= nl - 1 ? u3bb : 0.0f);
         }
         fprev = fk;
         prev = cur;

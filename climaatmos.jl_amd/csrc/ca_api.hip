@@ -677,7 +677,8 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     if (!geom || !par || !cfg || !out) return CA_ERR_ARG;
     *out = nullptr;
     const int nl = geom->nlev, nc = geom->ncol;
-    if (nl < 3 || nc < 1) return CA_ERR_ARG;
+    // kernels address a level row with 32-bit lane byte offsets: 4 * ncol < 2^31
+    if (nl < 3 || nc < 1 || nc > (1 << 28)) return CA_ERR_ARG;
     if (geom->nelem > 0 && nc != 16 * geom->nelem) return CA_ERR_ARG;
     if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3) return CA_ERR_ARG;
     if (cfg->vert_diff && !cfg->D0_col) return CA_ERR_ARG;

@@ -150,14 +150,15 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
     // (V4: the 4-node groups tid + 256 j of every plane)
     constexpr int NJ = V4 ? (CA_DSS_EMAX * 4 + 255) / 256 : (CA_DSS_EMAX * 16 + 255) / 256;
     constexpr int W = V4 ? 4 : 1;
-    int64_t soff[NJ];
+    // byte offsets of this thread's staged entries in a level row (-1: none)
+    int32_t soff[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         const int i = tid + 256 * j;
         if (V4)
-            soff[j] = i < nS / 4 ? 16 * (int64_t)list[i >> 2] + 4 * (i & 3) : -1;
+            soff[j] = i < nS / 4 ? 4 * (16 * list[i >> 2] + 4 * (i & 3)) : -1;
         else
-            soff[j] = i < nS ? 16 * (int64_t)list[i >> 4] + (i & 15) : -1;
+            soff[j] = i < nS ? 4 * (16 * list[i >> 4] + (i & 15)) : -1;
     }
     float pre[NJ][NP][W];
     auto fetch = [&](int k) {
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
                     for (int w = 0; w < W; ++w) pre[j][p][w] = 0.0f;
                     continue;
                 }
-                const float *a = src + p * nC + row + soff[j];
+                const char *a = reinterpret_cast<const char *>(src + p * nC + row) + (uint32_t)soff[j];
                 if constexpr (V4) {
                     const float4 v = *reinterpret_cast<const float4 *>(a);
                     pre[j][p][0] = v.x;
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
                     pre[j][p][2] = v.z;
                     pre[j][p][3] = v.w;
                 } else {
-                    pre[j][p][0] = *a;
+                    pre[j][p][0] = *reinterpret_cast<const float *>(a);
                 }
             }
         }
@@ -205,7 +206,7 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
         __syncthreads();
         const int64_t row = (int64_t)k * nc;
         if (col >= 0) {
-            const int64_t q = row + col;
+            const uint32_t cb = (uint32_t)col * 4u;
             if (k < nl) {
 #pragma unroll
                 for (int p = 0; p < NC; ++p) {
@@ -218,13 +219,13 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
                             o1 = fmaf(M[t].x, a, fmaf(M[t].y, b, o1));
                             o2 = fmaf(M[t].z, a, fmaf(M[t].w, b, o2));
                         }
-                        dst[p * nC + q] = o1;
-                        dst[(p + 1) * nC + q] = o2;
+                        ca_sto(dst + p * nC + row, cb, o1);
+                        ca_sto(dst + (p + 1) * nC + row, cb, o2);
                     } else {
                         float s = 0.0f;
 #pragma unroll
                         for (int t = 0; t < 4; ++t) s = fmaf(mw[t], sh[buf][p][sl[t]], s);
-                        dst[p * nC + q] = s;
+                        ca_sto(dst + p * nC + row, cb, s);
                     }
                 }
             }
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
 #pragma unroll
                 for (int t = 0; t < 4; ++t) s = fmaf(mw[t], sh[buf][NC][sl[t]], s);
                 if (filter && (k == 0 || k == nl)) s = 0.0f;
-                dst[NC * nC + q] = s;
+                ca_sto(dst + NC * nC + row, cb, s);
             }
         }
         buf ^= 1;

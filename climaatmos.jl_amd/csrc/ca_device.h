@@ -236,33 +236,52 @@ __device__ __forceinline__ float ca_elem0(float v) { return ca_dpp<0x150>(v); }
 #define CA_QB(m) ((m) | ((m) << 2) | ((m) << 4) | ((m) << 6))
 // row rotate right by r lanes (within the 16-lane row)
 #define CA_ROR(r) (0x120 + (r))
+// Each derivative is one block of four VALU instructions: a multiply and three
+// v_fmac_f32_dpp, each moving f by its DPP pattern in the operand fetch.  The
+// compiler folds a DPP move into v_mul / v_add but not into v_fmac (its
+// accumulator is tied to the destination), which would cost a separate
+// v_mov_b32_dpp per term, and these operators are VALU-issue bound.  Every
+// product and sum rounds as in fmaf(c[m], f(m), r) chained from m = 0 to 3.
+// The s_nop 1 ahead of the block is the two wait states a DPP read of a VGPR
+// needs after a VALU write of it (the compiler's hazard recognizer does not
+// look inside inline assembly); f is not written inside the block, so one
+// suffices.  The result is early-clobber: it is written before f and the
+// coefficients are last read.
+#define CA_DPP_MASKS " row_mask:0xf bank_mask:0xf bound_ctrl:1"
+#define CA_QUAD(m) " quad_perm:[" #m "," #m "," #m "," #m "]" CA_DPP_MASKS "\n\t"
+// sum_m D[i][m] f(m, j)   (c = q.c1) and sum_m D[m][i] g(m, j)   (c = q.c1t)
+__device__ __forceinline__ float ca_dquad(const float c[4], float f) {
+    float r;
+    asm("s_nop 1\n\t"
+        "v_mul_f32_dpp %0, %1, %2" CA_QUAD(0)
+        "v_fmac_f32_dpp %0, %1, %3" CA_QUAD(1)
+        "v_fmac_f32_dpp %0, %1, %4" CA_QUAD(2)
+        "v_fmac_f32_dpp %0, %1, %5" CA_QUAD(3)
+        : "=&v"(r)
+        : "v"(f), "v"(c[0]), "v"(c[1]), "v"(c[2]), "v"(c[3]));
+    return r;
+}
+// sum_k c[k] f(i, (j - k) & 3): rotation of the row by 4k lanes
+__device__ __forceinline__ float ca_dror(const float c[4], float f) {
+    float r;
+    asm("s_nop 1\n\t"
+        "v_mul_f32_e32 %0, %1, %2\n\t"
+        "v_fmac_f32_dpp %0, %1, %3 row_ror:4" CA_DPP_MASKS "\n\t"
+        "v_fmac_f32_dpp %0, %1, %4 row_ror:8" CA_DPP_MASKS "\n\t"
+        "v_fmac_f32_dpp %0, %1, %5 row_ror:12" CA_DPP_MASKS
+        : "=&v"(r)
+        : "v"(f), "v"(c[0]), "v"(c[1]), "v"(c[2]), "v"(c[3]));
+    return r;
+}
+#undef CA_QUAD
+#undef CA_DPP_MASKS
 // sum_m D[i][m] f(m, j)
-__device__ __forceinline__ float ca_d1(const Nod &q, float f) {
-    float r = q.c1[0] * ca_dpp<CA_QB(0)>(f);
-    r = fmaf(q.c1[1], ca_dpp<CA_QB(1)>(f), r);
-    r = fmaf(q.c1[2], ca_dpp<CA_QB(2)>(f), r);
-    return fmaf(q.c1[3], ca_dpp<CA_QB(3)>(f), r);
-}
+__device__ __forceinline__ float ca_d1(const Nod &q, float f) { return ca_dquad(q.c1, f); }
 // sum_m D[m][i] g(m, j)   (transpose: weak forms)
-__device__ __forceinline__ float ca_d1t(const Nod &q, float f) {
-    float r = q.c1t[0] * ca_dpp<CA_QB(0)>(f);
-    r = fmaf(q.c1t[1], ca_dpp<CA_QB(1)>(f), r);
-    r = fmaf(q.c1t[2], ca_dpp<CA_QB(2)>(f), r);
-    return fmaf(q.c1t[3], ca_dpp<CA_QB(3)>(f), r);
-}
+__device__ __forceinline__ float ca_d1t(const Nod &q, float f) { return ca_dquad(q.c1t, f); }
 // sum_m D[j][m] f(i, m): rotation by 4k brings f(i, (j - k) & 3)
-__device__ __forceinline__ float ca_d2(const Nod &q, float f) {
-    float r = q.c2[0] * f;
-    r = fmaf(q.c2[1], ca_dpp<CA_ROR(4)>(f), r);
-    r = fmaf(q.c2[2], ca_dpp<CA_ROR(8)>(f), r);
-    return fmaf(q.c2[3], ca_dpp<CA_ROR(12)>(f), r);
-}
-__device__ __forceinline__ float ca_d2t(const Nod &q, float f) {
-    float r = q.c2t[0] * f;
-    r = fmaf(q.c2t[1], ca_dpp<CA_ROR(4)>(f), r);
-    r = fmaf(q.c2t[2], ca_dpp<CA_ROR(8)>(f), r);
-    return fmaf(q.c2t[3], ca_dpp<CA_ROR(12)>(f), r);
-}
+__device__ __forceinline__ float ca_d2(const Nod &q, float f) { return ca_dror(q.c2, f); }
+__device__ __forceinline__ float ca_d2t(const Nod &q, float f) { return ca_dror(q.c2t, f); }
 
 // Metric factors of one node at one level: W = quadrature weight product,
 // J = 3-D Jacobian, and the reciprocals the operators multiply by.

@@ -78,7 +78,7 @@ EXPORTED = (
     "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint", "ca_step_record", "ca_step_pullback",
     "ca_set_column_params",
 )
-CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND, CA_TUNE_ADJ_CHUNK = 0, 1, 2, 3, 4
+CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND, CA_TUNE_ADJ_CHUNK, CA_TUNE_LAP_CHUNK = 0, 1, 2, 3, 4, 5
 CA_ENS_LANES, CA_ENS_ORDER = 0, 1
 _NOT_STATUS = ("ca_state_size", "ca_last_error", "ca_profile_count", "ca_ensemble_state_size",
                "ca_ensemble_member_offset", "ca_ensemble_last_error")

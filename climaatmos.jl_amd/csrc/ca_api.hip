@@ -575,11 +575,15 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     ars343(h);
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
     // level chunks sized so that small grids still fill the chip: about
-    // 1024 wavefronts for the per-node sweeps, 1024 workgroups for the DSS
+    // 1024 wavefronts for the per-node sweeps, 1024 workgroups for the DSS.
+    // The explicit tendency's cap of 6 levels: its wavefronts are long and
+    // few (5 per SIMD fit), so the last partial round of them costs; at
+    // h_elem 30, 63 levels, chunks of 6 (11 per column) measured 116 us
+    // against 127 us for chunks of 8 and 121 us for 9 to 11.
     {
         const int64_t layer_waves = (nc + 63) / 64;
         const int64_t want = (1024 + layer_waves - 1) / layer_waves;
-        h->explicit_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
+        h->explicit_chunk = (int)std::min<int64_t>(6, std::max<int64_t>(4, (nl + want - 1) / want));
         h->adj_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
         h->lap_chunk = (int)std::min<int64_t>(4, std::max<int64_t>(2, (nl + want - 1) / want));
         if (g.ncl > 0) {
@@ -905,6 +909,11 @@ ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value) {
         h->dss_kind = value;
         return CA_OK;
     }
+    if (knob == CA_TUNE_LAP_CHUNK) {
+        if (value < 1 || value > h->g.nlev) return fail(h, CA_ERR_ARG, "Laplacian chunk must be in [1, nlev]");
+        h->lap_chunk = value;
+        return CA_OK;
+    }
     if (knob == CA_TUNE_ADJ_CHUNK) {
         if (value < 1 || value > h->g.nlev) return fail(h, CA_ERR_ARG, "adjoint chunk must be in [1, nlev]");
         if (h->aw_d0 && value < h->adj_chunk)
@@ -922,6 +931,7 @@ ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value) {
     else if (knob == CA_TUNE_DSS_CHUNK) *value = h->dss_chunk;
     else if (knob == CA_TUNE_DSS_KIND) *value = h->dss_kind;
     else if (knob == CA_TUNE_ADJ_CHUNK) *value = h->adj_chunk;
+    else if (knob == CA_TUNE_LAP_CHUNK) *value = h->lap_chunk;
     else return CA_ERR_ARG;
     return CA_OK;
 }

@@ -144,6 +144,8 @@ ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *layouts
  *   CA_TUNE_ADJ_CHUNK:      levels per work item of the explicit-tendency
  *                           reverse (>= 1; the pullbacks' D0 partial sums
  *                           follow it).
+ *   CA_TUNE_LAP_CHUNK:      levels per work item of the Laplacian kernel
+ *                           (>= 1).
  * The chunk defaults follow the grid size (enough work items to fill the
  * GPU on small grids). */
 typedef enum ca_tune_knob {
@@ -151,7 +153,8 @@ typedef enum ca_tune_knob {
     CA_TUNE_EXPLICIT_CHUNK = 1,
     CA_TUNE_DSS_CHUNK = 2,
     CA_TUNE_DSS_KIND = 3,
-    CA_TUNE_ADJ_CHUNK = 4
+    CA_TUNE_ADJ_CHUNK = 4,
+    CA_TUNE_LAP_CHUNK = 5
 } ca_tune_knob;
 ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value);
 ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value);

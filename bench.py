@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--h-elem", type=int, default=30)
     ap.add_argument("--z-elem", type=int, default=63)
     ap.add_argument("--dt", type=float, default=60.0)
-    ap.add_argument("--cpu-baseline-steps", type=int, default=6)
+    ap.add_argument("--cpu-baseline-steps", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ragged column ensemble (512 columns, z_elem in {30, 45, 63, 90, 128}, 360 steps)")

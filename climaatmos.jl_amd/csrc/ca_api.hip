@@ -186,11 +186,10 @@ ca_status filter_columns(ca_handle *h, const float *src, float *dst, hipStream_t
     return check_launch(h, "filter");
 }
 
-// grid of the LDS-staged DSS: (cluster, level chunk) pairs, padded to a
-// multiple of 8 for the XCD remap
+// grid of the LDS-staged DSS: (cluster, level chunk) pairs, each of the 8
+// XCDs taking ceil(ncl / 8) clusters for every level chunk
 inline unsigned dss_lds_grid(const DevGeom &g, int chunk) {
-    const unsigned nb = (unsigned)g.ncl * cdiv(g.nlev, chunk);
-    return 8 * cdiv(nb, 8);
+    return 8 * cdiv(g.ncl, 8) * cdiv(g.nlev, chunk);
 }
 
 // DSS of NC centre planes (vector pair at VEC) and, with FACE, a face plane,

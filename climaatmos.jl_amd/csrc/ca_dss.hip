@@ -37,6 +37,23 @@ __device__ __forceinline__ bool xcd_block(int nb_col, int nb_lev, int &cb, int &
     return true;
 }
 
+// The LDS-staged kernel's grid: every XCD takes a contiguous range of
+// ceil(ncl / 8) element clusters, a compact patch of the sphere, for every
+// level chunk, in level-chunk-major order.  The workgroups in flight on one
+// XCD are then spatial neighbours at the same levels, so the element rows
+// one cluster stages as halo are, mostly, rows its neighbour staged
+// moments before through the same L2; only the patch's perimeter is read
+// from another XCD's side.  The host launches 8 ceil(ncl / 8) nb_lev
+// blocks; those mapped past the last cluster return.
+__device__ __forceinline__ bool xcd_cluster_block(int ncl, int nb_lev, int &cl, int &lb) {
+    const int cpx = (ncl + 7) >> 3;
+    const int b = blockIdx.x;
+    const int j = b >> 3;
+    lb = j / cpx;
+    cl = (b & 7) * cpx + (j - lb * cpx);
+    return lb < nb_lev && cl < ncl;
+}
+
 // weighted sum over the four slots of one plane row, in slot order
 __device__ __forceinline__ float dss_scalar_row(const float *__restrict__ row, const int mc[4], const float mw[4]) {
     float v[4];
@@ -129,7 +146,7 @@ __global__ void __launch_bounds__(256) k_dss_lds(DevGeom g, const float *__restr
     __shared__ float sh[2][NP][SE];
     const int nl = g.nlev;
     int cl, lb;
-    if (!xcd_block(g.ncl, (nl + chunk - 1) / chunk, cl, lb)) return;  // uniform per workgroup
+    if (!xcd_cluster_block(g.ncl, (nl + chunk - 1) / chunk, cl, lb)) return;  // uniform per workgroup
     const int tid = threadIdx.x;
     const int64_t nC = g.nC, nc = g.ncol;
     const int nS = 16 * g.dl_nE[cl];

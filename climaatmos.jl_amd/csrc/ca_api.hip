@@ -587,7 +587,9 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
         h->lap_chunk = (int)std::min<int64_t>(4, std::max<int64_t>(2, (nl + want - 1) / want));
         if (g.ncl > 0) {
             const int64_t wantd = (1024 + g.ncl - 1) / g.ncl;
-            h->dss_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(2, (nl + wantd - 1) / wantd));
+            // 6 levels per workgroup measured best at h_elem 30, 63 levels (the
+            // staged DSS's three launches 2-3% faster than with 8)
+            h->dss_chunk = (int)std::min<int64_t>(6, std::max<int64_t>(2, (nl + wantd - 1) / wantd));
         }
     }
     return CA_OK;

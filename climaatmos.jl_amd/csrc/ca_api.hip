@@ -575,14 +575,14 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
     // level chunks sized so that small grids still fill the chip: about
     // 1024 wavefronts for the per-node sweeps, 1024 workgroups for the DSS.
-    // The explicit tendency's cap of 6 levels: its wavefronts are long and
-    // few (5 per SIMD fit), so the last partial round of them costs; at
-    // h_elem 30, 63 levels, chunks of 6 (11 per column) measured 116 us
-    // against 127 us for chunks of 8 and 121 us for 9 to 11.
+    // The explicit tendency's cap of 7 levels: its wavefronts are long and
+    // few (6 per SIMD fit), so the last partial round of them costs; at
+    // h_elem 30, 63 levels, chunks of 7 (9 per column) fill the slots almost
+    // exactly twice: 112.6 us against 114-115 us for chunks of 5 or 8.
     {
         const int64_t layer_waves = (nc + 63) / 64;
         const int64_t want = (1024 + layer_waves - 1) / layer_waves;
-        h->explicit_chunk = (int)std::min<int64_t>(6, std::max<int64_t>(4, (nl + want - 1) / want));
+        h->explicit_chunk = (int)std::min<int64_t>(7, std::max<int64_t>(4, (nl + want - 1) / want));
         h->adj_chunk = (int)std::min<int64_t>(8, std::max<int64_t>(4, (nl + want - 1) / want));
         h->lap_chunk = (int)std::min<int64_t>(4, std::max<int64_t>(2, (nl + want - 1) / want));
         if (g.ncl > 0) {

@@ -69,9 +69,16 @@ struct ColState {
 // face term of its top centre).  chunk = 0 means the whole column.
 // SPH / HYPER / VDIFF select the sphere (horizontal operators), the
 // hyperdiffusion and the vertical diffusion at compile time.
+// The sphere dynamics with hyperdiffusion and no vertical diffusion (the
+// large-grid case) is compiled for 6 wavefronts per SIMD (80 VGPRs, a 20-byte
+// spill) instead of the 5 that 93 VGPRs allow: its wavefronts are long and
+// few, so how evenly they fill the slots matters more than the spill.  With
+// 7-level chunks (the default for large grids) the grid fills 6 per SIMD
+// almost exactly twice; at h_elem 30, 63 levels this measured 112.7 us
+// against 116.1 us for 5 per SIMD at its best chunk (6).
 template <bool SPH, bool HYPER, bool VDIFF>
-__global__ void __launch_bounds__(256) k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt,
-                                                  int chunk) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH && HYPER && !VDIFF ? 6 : 1, 8)))
+k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk) {
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.ncol) return;
     const Nod q = ca_node(threadIdx.x & 63);

@@ -61,11 +61,16 @@ class Oracle:
     """One oracle context: grid + constants + numerics.
 
     ``f32=True`` loads the Float32/OpenMP build (CPU timing baseline only).
+    ``lib_path`` loads another build of the same source (the pinning tests'
+    mutation run plants a mistake in a copy of it to show the checks catch it).
     """
 
-    def __init__(self, grid, params, numerics, alpha_w_col=None, D0_col=None, f32=False):
-        build()
-        lib = C.CDLL(os.path.join(_BUILD, "liboracle_f32.so" if f32 else "liboracle.so"))
+    def __init__(self, grid, params, numerics, alpha_w_col=None, D0_col=None, f32=False,
+                 lib_path=None):
+        if lib_path is None:
+            build()
+            lib_path = os.path.join(_BUILD, "liboracle_f32.so" if f32 else "liboracle.so")
+        lib = C.CDLL(lib_path)
         self.lib = lib
         self.f32 = f32
         self.dtype = np.float32 if f32 else np.float64

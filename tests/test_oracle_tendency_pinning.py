@@ -1,0 +1,295 @@
+"""Pins the oracle's explicit tendency (remaining_tendency!) against what the
+mathematics fixes, term by term, so that a wrong sign or a dropped term in the
+restatement fails here.
+
+* Horizontal dynamics and vertical advection
+  (src/prognostic_equations/advection.jl:36-91, :205-290): manufactured smooth
+  states on the deep cubed sphere; the oracle's DSSed tendency converges to the
+  analytic continuous tendency -- third order in the element size for the
+  horizontal terms, second order in the layer thickness for the vertical ones.
+* Hyperdiffusion (hyperdiffusion.jl:116-147 prep, :247-307 apply): degree-3
+  spherical harmonics are eigenfunctions; the mass-weighted decay rate equals
+  nu_v (l(l+1)/r^2)^2 for a rotational wind, delta times that for a divergent
+  wind, nu_s for the scalar (with rho inside the outer divergence) and nu_v for
+  u_3.
+* Vertical diffusion (vertical_diffusion_boundary_layer.jl:63-103): exact
+  discrete eigenmodes of the zero-flux second difference, the exact product
+  rule for linear rho and s_d, column conservation, and second-order
+  convergence on the deep sphere.
+* Rayleigh sponge on u_h (rayleigh_sponge.jl:39-44): -alpha zeta(z) u_h.
+
+Each check is a function of an Oracle factory so that
+tests/test_oracle_mutants.py can run the same checks on copies of the oracle
+with a planted mistake.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import manufactured as M
+from cadycore import geometry as G
+from cadycore import params as P
+from oracle import Oracle
+
+PAR = P.Params.from_toml()
+
+
+def default_factory(grid, params, numerics, **kw):
+    return Oracle(grid, params, numerics, **kw)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _tendency(mk, grid, numerics, Y, dss=True, **kw):
+    o = mk(grid, PAR, numerics, **kw)
+    Y = Y.copy()
+    o.set_precomputed(Y)
+    Yt = o.remaining_tendency(Y)
+    if dss:
+        o.dss(Yt)
+    return Yt
+
+
+def _planes(grid, Yt):
+    nl, nc = grid.nlev, grid.ncol
+    nC = nl * nc
+    return [Yt[i * nC:(i + 1) * nC].reshape(nl, nc) for i in range(4)] + [Yt[4 * nC:].reshape(nl + 1, nc)]
+
+
+def _order(errs):
+    return [math.log2(errs[i] / errs[i + 1]) for i in range(len(errs) - 1)]
+
+
+# ------------------------------------------------- manufactured solutions
+
+def mms_horizontal(mk, h_elems=(3, 6, 12), nlev=4):
+    """Relative L2 errors [(rho, u1, u2, rhoe)] of the DSSed explicit tendency
+    against the continuous one, w = 0, hyperdiffusion off."""
+    sm = M.Smooth(PAR)
+    out = []
+    for h in h_elems:
+        g = G.cubed_sphere(h, G.uniform_faces(nlev, 30000.0), PAR.planet_radius, PAR.Omega)
+        got = _planes(g, _tendency(mk, g, P.Numerics(dt=100.0, hyperdiff=False), sm.state(g)))
+        ana = sm.explicit_tendency(g)
+        out.append([_rel(got[i], ana[i]) for i in range(4)])
+    return out
+
+
+def mms_vertical(mk, nlevs=(8, 16, 32), h_elem=4, w_amp=0.0):
+    """Relative L2 errors of the explicit u_3 tendency at interior faces under
+    vertical refinement (w = w_amp profile)."""
+    sm = M.Smooth(PAR, w_amp=w_amp)
+    out = []
+    for nl in nlevs:
+        g = G.cubed_sphere(h_elem, G.uniform_faces(nl, 30000.0), PAR.planet_radius, PAR.Omega)
+        got = _planes(g, _tendency(mk, g, P.Numerics(dt=100.0, hyperdiff=False), sm.state(g)))
+        ana = sm.explicit_tendency(g)
+        out.append(_rel(got[4][1:-1], ana[4][1:-1]))
+    return out
+
+
+def mms_with_w(mk, pairs=((3, 8), (6, 16), (12, 32)), w_amp=2.0):
+    """Relative L2 errors [(u1, u2, u3)] with a vertical velocity, refining the
+    elements and the layers together."""
+    sm = M.Smooth(PAR, w_amp=w_amp)
+    out = []
+    for h, nl in pairs:
+        g = G.cubed_sphere(h, G.uniform_faces(nl, 30000.0), PAR.planet_radius, PAR.Omega)
+        got = _planes(g, _tendency(mk, g, P.Numerics(dt=100.0, hyperdiff=False), sm.state(g)))
+        ana = sm.explicit_tendency(g)
+        out.append([_rel(got[1], ana[1]), _rel(got[2], ana[2]), _rel(got[4][1:-1], ana[4][1:-1])])
+    return out
+
+
+def check_mms_horizontal(mk):
+    e = np.array(mms_horizontal(mk))
+    for f in range(4):
+        order = _order(e[:, f])
+        assert min(order) > 2.7, f"field {f}: orders {order} errors {e[:, f]}"
+    assert e[-1, 0] < 3e-4 and e[-1, 3] < 3e-4, e[-1]
+    assert e[-1, 1] < 4e-5 and e[-1, 2] < 4e-5, e[-1]
+
+
+def check_mms_vertical(mk):
+    e = mms_vertical(mk)
+    assert min(_order(e)) > 1.9 and e[-1] < 1.5e-5, e
+
+
+def check_mms_with_w(mk):
+    e = np.array(mms_with_w(mk))
+    for f in range(3):
+        order = _order(e[:, f])
+        assert min(order) > 1.8, f"field {f}: orders {order} errors {e[:, f]}"
+    assert e[-1, 0] < 3e-3 and e[-1, 1] < 1e-3 and e[-1, 2] < 1.5e-4, e[-1]
+
+
+def test_mms_horizontal_dynamics_third_order():
+    check_mms_horizontal(default_factory)
+
+
+def test_mms_vertical_advection_second_order():
+    check_mms_vertical(default_factory)
+
+
+def test_mms_with_vertical_velocity_converges():
+    check_mms_with_w(default_factory)
+
+
+# -------------------------------------------------------- hyperdiffusion
+
+def hyperdiffusion_rates(mk, h_elem, nlev=6):
+    """Mass-weighted decay rate of each harmonic state's hyperdiffusion
+    tendency relative to the closed form (1 = exact)."""
+    _, w, _ = G.gll()
+    g = G.cubed_sphere(h_elem, G.uniform_faces(nlev, 30000.0), PAR.planet_radius, PAR.Omega)
+    WJ = np.tile(np.outer(w, w).reshape(-1), g.nelem) * g.J_h
+    on, off = P.Numerics(dt=100.0, hyperdiff=True), P.Numerics(dt=100.0, hyperdiff=False)
+    nu_v, nu_s = on.nu4(g.h)
+    rates = {}
+    for kind in ("rot", "div", "scalar", "w"):
+        Y, info = M.harmonic_state(g, PAR, kind)
+        d = _planes(g, _tendency(mk, g, on, Y) - _tendency(mk, g, off, Y))
+        exp = M.hyperdiffusion_closed_form(g, PAR, kind, info, nu_v, nu_s, on.div_damping)
+        if kind in ("rot", "div"):
+            u = (info["u1"], info["u2"])
+            rates[kind] = sum((WJ * d[1 + i] * u[i]).sum() for i in range(2)) / \
+                sum((WJ * exp[i] * u[i]).sum() for i in range(2))
+        elif kind == "scalar":
+            rates[kind] = (WJ * d[3] * exp).sum() / (WJ * exp * exp).sum()
+        else:
+            rates[kind] = (WJ * d[4][2:-2] * exp[2:-2]).sum() / (WJ * exp[2:-2] ** 2).sum()
+    return rates
+
+
+def check_hyperdiffusion(mk):
+    r8, r16 = hyperdiffusion_rates(mk, 8), hyperdiffusion_rates(mk, 16)
+    for kind, tol in (("rot", 1.5e-3), ("div", 3e-4), ("scalar", 1e-4), ("w", 1e-3)):
+        assert abs(r16[kind] - 1) < tol, (kind, r8[kind], r16[kind])
+        assert abs(r16[kind] - 1) <= abs(r8[kind] - 1) + 1e-5, (kind, r8[kind], r16[kind])
+
+
+def test_hyperdiffusion_eigenmode_decay_rates():
+    check_hyperdiffusion(default_factory)
+
+
+# ----------------------------------------------------- vertical diffusion
+
+def _column_state(zc, rho, T, u1):
+    K = 0.5 * u1 ** 2
+    rhoe = rho * (PAR.cv_d * (T - PAR.T_0) - PAR.R_d * PAR.T_0 + K + PAR.grav * zc)
+    return np.concatenate([rho, u1, 0 * u1, rhoe, np.zeros(len(zc) + 1)])
+
+
+def check_vertical_diffusion_column(mk):
+    N, H, D0 = 12, 3000.0, 7.0
+    g = G.columns(1, G.uniform_faces(N, H))
+    zc, dz = g.zc, H / N
+    on = P.Numerics(dt=10.0, hyperdiff=False, vert_diff=True)
+    off = P.Numerics(dt=10.0, hyperdiff=False)
+    D = np.array([D0])
+
+    def vd(Y):
+        return _tendency(mk, g, on, Y, dss=False, D0_col=D) - _tendency(mk, g, off, Y, dss=False)
+
+    m = 3
+    cosm = np.cos(np.pi * m * (np.arange(N) + 0.5) / N)
+    lam = 4.0 / dz ** 2 * math.sin(math.pi * m / (2 * N)) ** 2
+    # s_d = cp_d (T - T_0) + g z is a cosine mode of the zero-flux second difference
+    T = 300.0 - PAR.grav / PAR.cp_d * zc + 2.0 * cosm
+    d = vd(_column_state(zc, np.full(N, 1.1), T, np.zeros(N)))
+    exp = -D0 * lam * 1.1 * PAR.cp_d * 2.0 * cosm
+    assert np.abs(d[3 * N:4 * N] - exp).max() < 1e-10 * np.abs(exp).max()
+    # u_h cosine mode
+    u = 5.0 * cosm
+    d = vd(_column_state(zc, np.full(N, 1.1), np.full(N, 280.0), u))
+    assert np.abs(d[N:2 * N] + D0 * lam * u).max() < 1e-10 * np.abs(D0 * lam * u).max()
+    # linear rho and linear s_d: interior rho e_t = D0 (d rho/dz)(d s_d/dz) exactly
+    rho, T = 1.2 - 1e-4 * zc, 300.0 - 0.003 * zc
+    d = vd(_column_state(zc, rho, T, np.zeros(N)))
+    exp = D0 * (-1e-4) * (PAR.cp_d * (-0.003) + PAR.grav)
+    np.testing.assert_allclose(d[3 * N + 1:4 * N - 1], exp, rtol=1e-9)
+    # zero-flux boundaries: column energy and momentum conserved
+    rng = np.random.default_rng(0)
+    Y = _column_state(zc, 1 + 0.1 * rng.random(N), 280 + 5 * rng.random(N), 3 * rng.standard_normal(N))
+    d = vd(Y)
+    scale = np.abs(d[3 * N:4 * N]).max()
+    assert abs(d[3 * N:4 * N].sum()) < 1e-12 * scale * N
+    assert abs((d[N:2 * N] * Y[:N]).sum()) < 1e-12 * np.abs(d[N:2 * N] * Y[:N]).max() * N
+
+
+def vertical_diffusion_sphere_errors(mk, nlevs=(8, 16, 32), D0=20.0):
+    sm = M.Smooth(PAR)
+    out = []
+    for nl in nlevs:
+        g = G.cubed_sphere(2, G.uniform_faces(nl, 30000.0), PAR.planet_radius, PAR.Omega)
+        on = P.Numerics(dt=100.0, hyperdiff=False, vert_diff=True)
+        off = P.Numerics(dt=100.0, hyperdiff=False)
+        Y = sm.state(g)
+        d = _planes(g, _tendency(mk, g, on, Y, dss=False, D0_col=np.full(g.ncol, D0))
+                    - _tendency(mk, g, off, Y, dss=False))
+        ex = M.vdiff_tendency(sm, g, D0)
+        out.append([_rel(d[3][1:-1], ex[0][1:-1]), _rel(d[1][1:-1], ex[1][1:-1]),
+                    _rel(d[2][1:-1], ex[2][1:-1])])
+    return out
+
+
+def check_vertical_diffusion_sphere(mk):
+    e = np.array(vertical_diffusion_sphere_errors(mk))
+    for f in range(3):
+        assert min(_order(e[:, f])) > 1.9 and e[-1, f] < 3e-3, e
+
+
+def test_vertical_diffusion_column_closed_forms():
+    check_vertical_diffusion_column(default_factory)
+
+
+def test_vertical_diffusion_deep_sphere_second_order():
+    check_vertical_diffusion_sphere(default_factory)
+
+
+# --------------------------------------------------------------- Rayleigh
+
+def check_rayleigh_uh(mk):
+    zf = G.uniform_faces(10, 30000.0)
+    g = G.cubed_sphere(2, zf, PAR.planet_radius, PAR.Omega)
+    Y = M.Smooth(PAR).state(g)
+    alpha = PAR.alpha_rayleigh_uh if PAR.alpha_rayleigh_uh != 0 else 1e-3
+
+    class _P:  # the same constants with a nonzero u_h coefficient
+        pass
+
+    par = _P()
+    par.__dict__.update(PAR.__dict__)
+    par.alpha_rayleigh_uh = alpha
+    on = P.Numerics(dt=100.0, hyperdiff=False, rayleigh=True)
+    off = P.Numerics(dt=100.0, hyperdiff=False)
+    o_on, o_off = mk(g, par, on), mk(g, par, off)
+    a, b = Y.copy(), Y.copy()
+    o_on.set_precomputed(a)
+    o_off.set_precomputed(b)
+    d = _planes(g, o_on.remaining_tendency(a) - o_off.remaining_tendency(b))
+    u = _planes(g, Y)
+    zmax, zd = zf[-1], PAR.zd_rayleigh
+    zeta = np.array([math.sin(math.pi / 2 * (z - zd) / (zmax - zd)) ** 2 if z > zd else 0.0 for z in g.zc])
+    for i in (1, 2):
+        exp = -alpha * zeta[:, None] * u[i]
+        assert np.abs(d[i] - exp).max() <= 1e-12 * np.abs(exp).max() + 1e-300, i
+    assert np.abs(d[0]).max() == 0 and np.abs(d[3]).max() == 0
+
+
+def test_rayleigh_uh_closed_form():
+    check_rayleigh_uh(default_factory)
+
+
+ALL_CHECKS = {
+    "mms_horizontal": check_mms_horizontal,
+    "mms_vertical": check_mms_vertical,
+    "mms_with_w": check_mms_with_w,
+    "hyperdiffusion": check_hyperdiffusion,
+    "vdiff_column": check_vertical_diffusion_column,
+    "vdiff_sphere": check_vertical_diffusion_sphere,
+    "rayleigh_uh": check_rayleigh_uh,
+}

@@ -76,7 +76,7 @@ EXPORTED = (
     "ca_ensemble_last_error", "ca_ensemble_step", "ca_ensemble_step_graph", "ca_ensemble_set_tuning",
     "ca_ensemble_get_tuning", "ca_ensemble_autotune", "ca_ensemble_set_params", "ca_ensemble_adjoint",
     "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint", "ca_step_record", "ca_step_pullback",
-    "ca_set_column_params",
+    "ca_set_column_params", "ca_set_debug_poison",
 )
 CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND, CA_TUNE_ADJ_CHUNK, CA_TUNE_LAP_CHUNK = 0, 1, 2, 3, 4, 5
 CA_ENS_LANES, CA_ENS_ORDER = 0, 1
@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH):
     lib.ca_unpack_state.argtypes = [vp, vp, C.POINTER(CaLayout), vp, vp]
     lib.ca_check_finite.argtypes = [vp, vp, C.POINTER(C.c_int32), vp]
     lib.ca_profile_enable.argtypes = [vp, C.c_int32]
+    lib.ca_set_debug_poison.argtypes = [vp, C.c_int32]
     lib.ca_profile_count.argtypes = []
     lib.ca_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
     lib.ca_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
@@ -144,6 +145,52 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def create_args(grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None, D0_col=None):
+    """The ca_geometry / ca_params / ca_config structs of ca_create for a grid,
+    and the arrays they point into (keep them alive while the structs are used)."""
+    keep = {}
+
+    def arr(name, a, dt=np.float64):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep[name] = a
+        return a
+
+    _, wq, D = gll()
+    g = CaGeometry()
+    g.nelem, g.ncol, g.nlev, g.radius = grid.nelem, grid.ncol, grid.nlev, grid.radius
+    g.z_face = _dp(arr("z", grid.z_face))
+    g.J_h = _dp(arr("J", grid.J_h))
+    g.gup = _dp(arr("gup", grid.gup))
+    g.gdn = _dp(arr("gdn", grid.gdn))
+    g.f3_num = _dp(arr("f3", grid.f3_num))
+    g.f12 = _dp(arr("f12", grid.f12))
+    g.basis = _dp(arr("b", grid.basis))
+    g.n_dss = len(grid.dss_ptr) - 1
+    g.dss_ptr = arr("dp", grid.dss_ptr, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+    g.dss_col = arr("dc", grid.dss_col, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+    g.dss_w = _dp(arr("dw", grid.dss_w))
+    g.gll_w = _dp(arr("w", wq))
+    g.gll_D = _dp(arr("D", D))
+    p = CaParams(params.R_d, params.cp_d, params.cv_d, params.T_0, params.p_ref_theta, params.grav,
+                 params.T_surf_ref, params.T_min_ref, params.T_min_sgs)
+    c = CaConfig()
+    c.hyperdiff = int(bool(numerics.hyperdiff) and grid.nelem > 0)
+    nu_v, nu_s = numerics.nu4(grid.h) if grid.nelem > 0 else (0.0, 0.0)
+    c.nu4_vort, c.nu4_scal, c.div_damp = nu_v, nu_s, numerics.div_damping
+    c.upwinding = int(numerics.upwinding)
+    c.rayleigh = int(bool(numerics.rayleigh))
+    c.zd_rayleigh, c.alpha_uh, c.alpha_w = params.zd_rayleigh, params.alpha_rayleigh_uh, params.alpha_rayleigh_w
+    if alpha_w_col is not None:
+        c.alpha_w_col = _dp(arr("aw", alpha_w_col))
+    c.vert_diff = int(bool(numerics.vert_diff))
+    if numerics.vert_diff:
+        if D0_col is None:
+            raise ValueError("vertical diffusion needs D0_col")
+        c.D0_col = _dp(arr("D0", D0_col))
+    c.dt = numerics.dt
+    return g, p, c, keep
+
+
 class DryDycore:
     """One native dry-dycore instance on the current GPU.
 
@@ -163,46 +210,7 @@ class DryDycore:
         torch.cuda.set_device(self.device)
         self.lib = load_library(lib_path)
         self.grid, self.params, self.numerics = grid, params, numerics
-        keep = {}
-
-        def arr(name, a, dt=np.float64):
-            a = np.ascontiguousarray(a, dtype=dt)
-            keep[name] = a
-            return a
-
-        _, wq, D = gll()
-        g = CaGeometry()
-        g.nelem, g.ncol, g.nlev, g.radius = grid.nelem, grid.ncol, grid.nlev, grid.radius
-        g.z_face = _dp(arr("z", grid.z_face))
-        g.J_h = _dp(arr("J", grid.J_h))
-        g.gup = _dp(arr("gup", grid.gup))
-        g.gdn = _dp(arr("gdn", grid.gdn))
-        g.f3_num = _dp(arr("f3", grid.f3_num))
-        g.f12 = _dp(arr("f12", grid.f12))
-        g.basis = _dp(arr("b", grid.basis))
-        g.n_dss = len(grid.dss_ptr) - 1
-        g.dss_ptr = arr("dp", grid.dss_ptr, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
-        g.dss_col = arr("dc", grid.dss_col, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
-        g.dss_w = _dp(arr("dw", grid.dss_w))
-        g.gll_w = _dp(arr("w", wq))
-        g.gll_D = _dp(arr("D", D))
-        p = CaParams(params.R_d, params.cp_d, params.cv_d, params.T_0, params.p_ref_theta, params.grav,
-                     params.T_surf_ref, params.T_min_ref, params.T_min_sgs)
-        c = CaConfig()
-        c.hyperdiff = int(bool(numerics.hyperdiff) and grid.nelem > 0)
-        nu_v, nu_s = numerics.nu4(grid.h) if grid.nelem > 0 else (0.0, 0.0)
-        c.nu4_vort, c.nu4_scal, c.div_damp = nu_v, nu_s, numerics.div_damping
-        c.upwinding = int(numerics.upwinding)
-        c.rayleigh = int(bool(numerics.rayleigh))
-        c.zd_rayleigh, c.alpha_uh, c.alpha_w = params.zd_rayleigh, params.alpha_rayleigh_uh, params.alpha_rayleigh_w
-        if alpha_w_col is not None:
-            c.alpha_w_col = _dp(arr("aw", alpha_w_col))
-        c.vert_diff = int(bool(numerics.vert_diff))
-        if numerics.vert_diff:
-            if D0_col is None:
-                raise ValueError("vertical diffusion needs D0_col")
-            c.D0_col = _dp(arr("D0", D0_col))
-        c.dt = numerics.dt
+        g, p, c, keep = create_args(grid, params, numerics, alpha_w_col, D0_col)
         h = C.c_void_p()
         st = self.lib.ca_create(C.byref(g), C.byref(p), C.byref(c), C.byref(h))
         if st != CA_OK:
@@ -323,6 +331,10 @@ class DryDycore:
         flag = C.c_int32(0)
         st = self.lib.ca_check_finite(self.h, self._ptr(Y), C.byref(flag), self._stream())
         return st == CA_OK and flag.value == 0
+
+    def set_debug_poison(self, on=True):
+        """NaN-fill the handle's work buffers at every call (fill_with_nans!)."""
+        self._check(self.lib.ca_set_debug_poison(self.h, int(on)), "set_debug_poison")
 
     def set_tuning(self, knob, value):
         self._check(self.lib.ca_set_tuning(self.h, int(knob), int(value)), "set_tuning")

@@ -80,7 +80,17 @@ struct ca_handle {
     bool rec_valid = false;
     double rec_dt = 0.0;
     int rec_nsteps = 0, rec_every = 0;
+    // debug: fill the scratch no call carries over with NaN at every entry
+    // point (the counterpart of fill_with_nans!, utils/debug_utils.jl:67-70)
+    bool poison = false;
 };
+
+// 32-bit pattern fill of n words (NaN poisoning and zeroing; a kernel, so that
+// a caller capturing the stream into a hipGraph records no memset node)
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
 
 namespace {
 
@@ -104,6 +114,50 @@ ca_status dalloc(ca_handle *h, T **p, size_t count) {
     h->allocs.push_back(q);
     *p = (T *)q;
     return CA_OK;
+}
+
+// frees one of the handle's allocations (after the device is idle: kernels in
+// flight on any stream may still read it) and drops it from the list
+ca_status dfree(ca_handle *h, void *p) {
+    if (!p) return CA_OK;
+    HIPCHK(h, hipDeviceSynchronize());
+    for (size_t i = 0; i < h->allocs.size(); ++i)
+        if (h->allocs[i] == p) {
+            h->allocs.erase(h->allocs.begin() + i);
+            HIPCHK(h, hipFree(p));
+            return CA_OK;
+        }
+    return fail(h, CA_ERR_ARG, "dfree: not an allocation of this handle");
+}
+
+constexpr uint32_t kNaNBits = 0xffffffffu;  // a NaN as float, and its pairs a NaN as double
+
+ca_status fill_words(ca_handle *h, void *p, int64_t nwords, uint32_t v, hipStream_t st) {
+    if (!p || nwords <= 0) return CA_OK;
+    const int64_t blocks = std::min<int64_t>((nwords + 255) / 256, 4096);
+    k_fill_u32<<<(unsigned)blocks, 256, 0, st>>>((uint32_t *)p, nwords, v);
+    HIPCHK(h, hipGetLastError());
+    return CA_OK;
+}
+
+// Debug poisoning: the forward work buffers (no entry point carries them from
+// one call to the next) get NaN at every entry point while h->poison is set,
+// so that a read of a value the call did not write shows up as NaN.  The
+// Jacobian bands (update_jacobian -> ldiv) and the checkpoints and records
+// (step_record -> step_pullback) are poisoned only by the calls that write them.
+ca_status poison_work(ca_handle *h, hipStream_t st) {
+    if (!h->poison) return CA_OK;
+    const DevGeom &g = h->g;
+    ca_status s;
+    if ((s = fill_words(h, h->pre, 4 * g.nC, kNaNBits, st))) return s;
+    if ((s = fill_words(h, h->L, 4 * g.nC, kNaNBits, st))) return s;
+    if ((s = fill_words(h, h->scr, 2 * g.nF + 6 * g.nC, kNaNBits, st))) return s;
+    for (int i = 0; i < 4; ++i)
+        if ((s = fill_words(h, h->FE[i], h->S, kNaNBits, st))) return s;
+    for (int i = 1; i < 4; ++i)
+        if ((s = fill_words(h, h->FI[i], 2 * g.nC + g.nF, kNaNBits, st))) return s;
+    if ((s = fill_words(h, h->U, h->S, kNaNBits, st))) return s;
+    return fill_words(h, h->raw, h->S, kNaNBits, st);
 }
 
 template <class T>
@@ -571,6 +625,14 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     if ((s = dalloc(h, &h->raw, (size_t)h->S))) return s;
     if ((s = dalloc(h, &h->flag, 1))) return s;
     HIPCHK(h, hipMemset(h->L, 0, 4 * (size_t)g.nC * sizeof(float)));
+    // the work buffers start as NaN (all bits set): a read of a value no
+    // kernel wrote comes out non-finite instead of as stale memory
+    for (float *p : {h->pre, h->U, h->raw})
+        HIPCHK(h, hipMemset(p, 0xff, (p == h->pre ? 4 * (size_t)g.nC : (size_t)h->S) * sizeof(float)));
+    HIPCHK(h, hipMemset(h->scr, 0xff, (2 * (size_t)g.nF + 6 * (size_t)g.nC) * sizeof(float)));
+    for (int i = 0; i < 4; ++i) HIPCHK(h, hipMemset(h->FE[i], 0xff, (size_t)h->S * sizeof(float)));
+    for (int i = 1; i < 4; ++i)
+        HIPCHK(h, hipMemset(h->FI[i], 0xff, (2 * (size_t)g.nC + (size_t)g.nF) * sizeof(float)));
     ars343(h);
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
     // level chunks sized so that small grids still fill the chip: about
@@ -593,6 +655,55 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
         }
     }
     return CA_OK;
+}
+
+// The Nq = 4 Gauss-Lobatto-Legendre weights and derivative matrix: the
+// kernels keep W, 1/W and D in __constant__ memory shared by every handle of
+// the process, so ca_create accepts these tables only.
+bool is_gll4(const double *w, const double *D) {
+    const double x[4] = {-1.0, -1.0 / std::sqrt(5.0), 1.0 / std::sqrt(5.0), 1.0};
+    const double wq[4] = {1.0 / 6.0, 5.0 / 6.0, 5.0 / 6.0, 1.0 / 6.0};
+    double bw[4];
+    for (int k = 0; k < 4; ++k) {
+        double p = 1.0;
+        for (int m = 0; m < 4; ++m)
+            if (m != k) p *= x[k] - x[m];
+        bw[k] = 1.0 / p;
+    }
+    for (int i = 0; i < 4; ++i) {
+        if (std::fabs(w[i] - wq[i]) > 1e-12) return false;
+        double diag = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            if (k == i) continue;
+            const double d = (bw[k] / bw[i]) / (x[i] - x[k]);
+            diag -= d;
+            if (std::fabs(D[4 * i + k] - d) > 1e-10) return false;
+        }
+        if (std::fabs(D[4 * i + i] - diag) > 1e-10) return false;
+    }
+    return true;
+}
+
+// ca_create's argument checks; an empty string when they pass
+std::string check_create_args(const ca_geometry *geom, const ca_params *par, const ca_config *cfg) {
+    if (!geom || !par || !cfg) return "null geometry, params or config";
+    const int nl = geom->nlev, nc = geom->ncol;
+    // kernels address a level row with 32-bit lane byte offsets: 4 * ncol < 2^31
+    if (nl < 3 || nc < 1 || nc > (1 << 28)) return "need nlev >= 3 and 1 <= ncol <= 2^28";
+    if (geom->nelem < 0 || (geom->nelem > 0 && nc != 16 * geom->nelem)) return "ncol must be 16 * nelem on a sphere";
+    if (!geom->z_face || !geom->J_h || !geom->gup || !geom->gdn || !geom->f3_num || !geom->f12 || !geom->gll_w ||
+        !geom->gll_D)
+        return "null geometry table (z_face, J_h, gup, gdn, f3_num, f12, gll_w and gll_D are required)";
+    if (geom->nelem > 0 && (!geom->basis || !geom->dss_ptr || geom->n_dss < 0 ||
+                            (geom->n_dss > 0 && (!geom->dss_col || !geom->dss_w))))
+        return "a sphere needs basis, dss_ptr, dss_col and dss_w";
+    for (int k = 0; k < nl; ++k)
+        if (!(geom->z_face[k + 1] > geom->z_face[k])) return "z_face must increase strictly";
+    if (!is_gll4(geom->gll_w, geom->gll_D)) return "gll_w / gll_D must be the Nq = 4 GLL weights and derivative matrix";
+    if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3)
+        return "upwinding must be 0 (none), 1 (first_order) or 3 (vanleer_limiter)";
+    if (cfg->vert_diff && !cfg->D0_col) return "vertical diffusion needs D0_col";
+    return std::string();
 }
 
 // Records of one step for the reverse pass: U_0 (the step's start state),
@@ -679,14 +790,13 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
 extern "C" {
 
 ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_config *cfg, ca_handle **out) {
-    if (!geom || !par || !cfg || !out) return CA_ERR_ARG;
+    if (!out) return CA_ERR_ARG;
     *out = nullptr;
-    const int nl = geom->nlev, nc = geom->ncol;
-    // kernels address a level row with 32-bit lane byte offsets: 4 * ncol < 2^31
-    if (nl < 3 || nc < 1 || nc > (1 << 28)) return CA_ERR_ARG;
-    if (geom->nelem > 0 && nc != 16 * geom->nelem) return CA_ERR_ARG;
-    if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3) return CA_ERR_ARG;
-    if (cfg->vert_diff && !cfg->D0_col) return CA_ERR_ARG;
+    const std::string msg = check_create_args(geom, par, cfg);
+    if (!msg.empty()) {
+        g_create_error = "ca_create: " + msg;
+        return CA_ERR_ARG;
+    }
     ca_handle *h = new ca_handle();
     const ca_status s = create_impl(geom, par, cfg, h);
     if (s != CA_OK) {
@@ -717,6 +827,8 @@ ca_status ca_set_precomputed(ca_handle *h, float *Y, void *stream) {
     if (!h || !Y) return CA_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
+    ca_status s;
+    if ((s = poison_work(h, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
         k_precompute<<<dim3(cdiv(g.ncol, 128), g.nlev), 128, 0, st>>>(g, h->n, Y, h->pre, h->pre + g.nC,
@@ -727,6 +839,8 @@ ca_status ca_set_precomputed(ca_handle *h, float *Y, void *stream) {
 
 ca_status ca_remaining_tendency(ca_handle *h, float *Yt, const float *Y, void *stream) {
     if (!h || !Yt || !Y) return CA_ERR_ARG;
+    ca_status s;
+    if ((s = poison_work(h, (hipStream_t)stream))) return s;
     return explicit_tendency(h, Yt, Y, (hipStream_t)stream);
 }
 
@@ -734,6 +848,8 @@ ca_status ca_implicit_tendency(ca_handle *h, float *Yt, const float *Y, void *st
     if (!h || !Yt || !Y) return CA_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
+    ca_status s;
+    if ((s = poison_work(h, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
         k_implicit_tendency<<<dim3(cdiv(g.ncol, 128), g.nlev + 1), 128, 0, st>>>(g, h->n, Yt, Y);
@@ -747,6 +863,8 @@ ca_status ca_update_jacobian(ca_handle *h, const float *Y, double dtgamma, void 
     if (s) return s;
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
+    if ((s = poison_work(h, st))) return s;
+    if (h->poison && (s = fill_words(h, h->bands, 4 * g.nC + 11 * g.nF, kNaNBits, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
         k_update_jacobian<<<dim3(cdiv(g.ncol, 128), g.nlev + 1), 128, 0, st>>>(g, h->n, Y, (float)dtgamma, h->B);
@@ -759,6 +877,8 @@ ca_status ca_ldiv(ca_handle *h, float *dY, const float *R, void *stream) {
     if (!h->bands) return fail(h, CA_ERR_ARG, "ca_ldiv called before ca_update_jacobian");
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
+    ca_status s;
+    if ((s = poison_work(h, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
         k_ldiv<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, dY, R, h->scr);
@@ -770,6 +890,8 @@ ca_status ca_correct_implicit_advection(ca_handle *h, float *Yt, const float *Y,
     if (!h || !Yt || !Y) return CA_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
+    ca_status s;
+    if ((s = poison_work(h, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
         k_correct<<<dim3(cdiv(g.ncol, 128), g.nlev), 128, 0, st>>>(g, h->n, Yt, Y);
@@ -779,6 +901,8 @@ ca_status ca_correct_implicit_advection(ca_handle *h, float *Yt, const float *Y,
 
 ca_status ca_dss(ca_handle *h, float *Y, void *stream) {
     if (!h || !Y) return CA_ERR_ARG;
+    ca_status s;
+    if ((s = poison_work(h, (hipStream_t)stream))) return s;
     return dss_state(h, Y, (hipStream_t)stream, KID_DSS_STATE);
 }
 
@@ -787,6 +911,7 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
     ca_status s;
+    if ((s = poison_work(h, st))) return s;
     // the step starts from a continuous, filtered state (the reference's
     // state after dss! and cache! at the end of every step)
     if (g.nelem > 0) {
@@ -885,6 +1010,12 @@ ca_status ca_profile_enable(ca_handle *h, int32_t enable) {
 }
 
 int32_t ca_profile_count(void) { return KID_COUNT; }
+
+ca_status ca_set_debug_poison(ca_handle *h, int32_t on) {
+    if (!h) return CA_ERR_ARG;
+    h->poison = on != 0;
+    return CA_OK;
+}
 
 ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value) {
     if (!h) return CA_ERR_ARG;

@@ -15,6 +15,8 @@ ca_status adj_ensure(ca_handle *h, int64_t rec_steps, int64_t nck) {
     if (!h->aw_buf) {
         if ((s = dalloc(h, &h->aw_buf, (size_t)AB_COUNT * h->S))) return s;
         if ((s = dalloc(h, &h->aw_L, 16 * (size_t)g.nC))) return s;
+        HIPCHK(h, hipMemset(h->aw_buf, 0xff, (size_t)AB_COUNT * h->S * sizeof(float)));
+        HIPCHK(h, hipMemset(h->aw_L, 0xff, 16 * (size_t)g.nC * sizeof(float)));
         const int nch = (int)cdiv(g.nlev, h->adj_chunk);
         if ((s = dalloc(h, &h->aw_d0, (size_t)nch * g.ncol))) return s;
         // per-column one-column views of the geometry (the Newton stage reverse)
@@ -41,16 +43,42 @@ ca_status adj_ensure(ca_handle *h, int64_t rec_steps, int64_t nck) {
         }
         if ((s = upload(h, &h->cgeo, v))) return s;
     }
+    // the records and checkpoints grow with the call's `every` and step
+    // count; a larger request replaces the smaller buffer
     if (rec_steps > h->aw_rec_steps) {
+        if ((s = dfree(h, h->aw_rec))) return s;
+        h->aw_rec = nullptr;
+        h->aw_rec_steps = 0;
         if ((s = dalloc(h, &h->aw_rec, (size_t)(rec_steps * rec_size(h))))) return s;
+        HIPCHK(h, hipMemset(h->aw_rec, 0xff, (size_t)(rec_steps * rec_size(h)) * sizeof(float)));
         h->aw_rec_steps = rec_steps;
     }
     if (nck > h->aw_ck_n) {
+        if ((s = dfree(h, h->aw_ck))) return s;
+        h->aw_ck = nullptr;
+        h->aw_ck_n = 0;
         if ((s = dalloc(h, &h->aw_ck, (size_t)(nck * h->S)))) return s;
+        HIPCHK(h, hipMemset(h->aw_ck, 0xff, (size_t)(nck * h->S) * sizeof(float)));
         h->aw_ck_n = nck;
     }
     return CA_OK;
 }
+
+// Debug poisoning of the reverse-mode scratch (see poison_work); with
+// records, also the per-step records, which every pullback rewrites
+ca_status poison_reverse(ca_handle *h, hipStream_t st, bool records) {
+    if (!h->poison) return CA_OK;
+    ca_status s;
+    if ((s = poison_work(h, st))) return s;
+    if ((s = fill_words(h, h->aw_buf, (int64_t)AB_COUNT * h->S, kNaNBits, st))) return s;
+    if ((s = fill_words(h, h->aw_L, 16 * h->g.nC, kNaNBits, st))) return s;
+    if (records && (s = fill_words(h, h->aw_rec, h->aw_rec_steps * rec_size(h), kNaNBits, st))) return s;
+    return CA_OK;
+}
+
+// the Newton-stage reverse keeps a column's faces in one workgroup's LDS rows
+// of at most 129 entries
+constexpr int kMaxAdjLevels = 128;
 
 float *abuf(ca_handle *h, int i) { return h->aw_buf + (int64_t)i * h->S; }
 
@@ -158,6 +186,7 @@ const float *face_b(const ca_handle *h) { return h->aw_L + 12 * h->g.nC; }
 ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Un, const float *Ub, float dtg, float *Utb,
                      const float *fib, float cfi, const float *fa, const float *fb, hipStream_t st) {
     const DevGeom &g = h->g;
+    if (g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "newton_vjp: nlev > 128");
     if (g.nlev <= 32)
         sadj::k_newton_vjp<32, 33, 2><<<dim3(cdiv(g.ncol, 2)), 64, 0, st>>>(h->cgeo, h->n, g.nC, g.ncol, Ut, Un, Ub,
                                                                             dtg, Utb, fib, cfi, fa, fb);
@@ -172,8 +201,7 @@ ca_status newton_vjp(ca_handle *h, const float *Ut, const float *Un, const float
 
 ca_status zero_d0(ca_handle *h, hipStream_t st) {
     const int nch = (int)cdiv(h->g.nlev, h->adj_chunk);
-    HIPCHK(h, hipMemsetAsync(h->aw_d0, 0, (size_t)nch * h->g.ncol * sizeof(double), st));
-    return CA_OK;
+    return fill_words(h, h->aw_d0, 2 * (int64_t)nch * h->g.ncol, 0u, st);
 }
 
 ca_status sum_d0(ca_handle *h, float *D0b, hipStream_t st) {
@@ -234,6 +262,7 @@ ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     ca_status s;
     if ((s = adj_ensure(h, 0, 0))) return s;
+    if ((s = poison_reverse(h, st, false))) return s;
     float *tmp = abuf(h, AB_GB);
     HIPCHK(h, hipMemcpyAsync(tmp, Yb, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
     return dss_T_state(h, tmp, Yb, 0, st);
@@ -247,6 +276,7 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
     const DevGeom &g = h->g;
     ca_status s;
     if ((s = adj_ensure(h, 0, 0))) return s;
+    if ((s = poison_reverse(h, st, false))) return s;
     if (h->n.hyper) {  // the DSSed Laplacians at Y
         {
             Launch l(h, st, KID_LAPLACIANS);
@@ -268,12 +298,15 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
 ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int32_t every, void *stream) {
     if (!h || !Y || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_record: sphere grids only (columns: ca_ensemble_adjoint)");
+    if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode supports nlev <= 128");
     hipStream_t st = (hipStream_t)stream;
     const int64_t S = h->S;
     const int nseg = (nsteps + every - 1) / every;
     ca_status s;
     h->rec_valid = false;
     if ((s = adj_ensure(h, every, nseg))) return s;
+    if ((s = poison_reverse(h, st, true))) return s;
+    if (h->poison && (s = fill_words(h, h->aw_ck, h->aw_ck_n * S, kNaNBits, st))) return s;
     // the step's initial dss! and filter, then a checkpoint at every segment start
     HIPCHK(h, hipMemcpyAsync(h->U, Y, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
     if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
@@ -299,6 +332,7 @@ ca_status ca_step_pullback(ca_handle *h, const float *YbN, float *Yb0, float *D0
     const double dt = h->rec_dt;
     const int nseg = (nsteps + every - 1) / every;
     ca_status s;
+    if ((s = poison_reverse(h, st, true))) return s;
     float *Yw = abuf(h, AB_Y);
     if ((s = zero_d0(h, st))) return s;
     HIPCHK(h, hipMemcpyAsync(abuf(h, AB_UB), YbN, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -323,6 +357,7 @@ ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, doubl
                           float *Yb0, float *D0b, void *stream) {
     if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
+    if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode supports nlev <= 128");
     hipStream_t st = (hipStream_t)stream;
     ca_status s;
     if ((s = adj_ensure(h, every, (nsteps + every - 1) / every))) return s;

@@ -101,6 +101,13 @@ typedef struct ca_layout {
     int64_t offset, s_v, s_i, s_j, s_h;
 } ca_layout;
 
+/* ca_create checks its arguments and returns CA_ERR_ARG (message in
+ * ca_last_error(NULL)) on: null required tables (z_face, J_h, gup, gdn, f3_num,
+ * f12, gll_w, gll_D; on a sphere also basis and the dss_* arrays), nlev < 3,
+ * ncol outside [1, 2^28], ncol != 16 nelem on a sphere, z_face not strictly
+ * increasing, GLL tables other than the Nq = 4 Gauss-Lobatto-Legendre ones
+ * (the kernels share them process-wide), an unknown upwinding code, vertical
+ * diffusion without D0_col, or R_d / cp_d off 2/7. */
 ca_status ca_create(const ca_geometry *geom, const ca_params *params, const ca_config *config,
                     ca_handle **out);
 ca_status ca_destroy(ca_handle *h);
@@ -162,6 +169,16 @@ ca_status ca_get_tuning(const ca_handle *h, int32_t knob, int32_t *value);
 /* non-finite check (NaN poisoning counterpart): *flag = 1 if Y holds NaN/Inf.
  * Synchronises the stream. */
 ca_status ca_check_finite(ca_handle *h, const float *Y, int32_t *flag, void *stream);
+
+/* Debug poisoning (the counterpart of fill_with_nans!, utils/debug_utils.jl:67-70,
+ * which fills the cache with NaN before each tendency evaluation): while on,
+ * every entry point first fills the handle's work buffers that no call
+ * carries over with NaN, so a value read without being written in the same
+ * call shows up as NaN in the result.  The Jacobian bands are poisoned by
+ * ca_update_jacobian (which rewrites them; ca_ldiv reads them), the
+ * checkpoints by ca_step_record.  Results are unchanged when the library is
+ * correct; off by default. */
+ca_status ca_set_debug_poison(ca_handle *h, int32_t on);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * every kernel launch (off by default; enabling adds event overhead).
@@ -242,7 +259,9 @@ ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream);
  * With vertical diffusion and D0b != NULL, D0b [ncol] <- d <Ytb, T_exp> / d D0. */
 ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, const float *Y, float *D0b,
                                     void *stream);
-/* ca_step that also keeps what its pullback needs: runs nsteps steps of Y in
+/* The sphere reverse mode (ca_step_record, ca_step_pullback, ca_step_adjoint)
+ * supports nlev <= 128 and returns CA_ERR_ARG above that, before any launch.
+ * ca_step that also keeps what its pullback needs: runs nsteps steps of Y in
  * place (as ca_step) and stores the state at every `every`-th step in the
  * handle (ceil(nsteps / every) states; ca_step_pullback adds `every` steps of
  * stage records, 7 states + 4 Laplacian sets each). */

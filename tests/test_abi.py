@@ -62,3 +62,55 @@ def test_create_rejects_bad_arguments_without_touching_the_device(lib):
     h = ctypes.c_void_p()
     assert lib.ca_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(c), ctypes.byref(h)) == 1
     assert lib.ca_create(None, None, None, ctypes.byref(h)) == 1
+
+
+def test_create_validates_tables_before_touching_the_device(lib):
+    """Null tables, non-increasing faces and non-GLL quadrature tables are
+    rejected by ca_create's argument checks, which run before any HIP call."""
+    import numpy as np
+
+    from cadycore import geometry as G
+    from cadycore import lib as L
+    from cadycore import params as P
+
+    lib.ca_create.restype = ctypes.c_int
+    lib.ca_last_error.restype = ctypes.c_char_p
+    par = P.Params.from_toml()
+    grid = G.cubed_sphere(1, G.uniform_faces(4, 30000.0), par.planet_radius, par.Omega)
+
+    def create(edit):
+        g, p, c, keep = L.create_args(grid, par, P.Numerics())
+        edit(g, c, keep)
+        h = ctypes.c_void_p()
+        st = lib.ca_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(c), ctypes.byref(h))
+        return st, lib.ca_last_error(None).decode(), h
+
+    def null(field):
+        return lambda g, c, keep: setattr(g, field, None)
+
+    for f in ("z_face", "J_h", "gup", "gdn", "f3_num", "f12", "gll_w", "gll_D", "basis", "dss_ptr"):
+        st, msg, h = create(null(f))
+        assert st == 1 and h.value is None and "ca_create" in msg, (f, msg)
+
+    def flat_faces(g, c, keep):
+        z = keep["z"].copy()
+        z[2] = z[1]
+        keep["z2"] = z
+        g.z_face = L._dp(z)
+
+    st, msg, _ = create(flat_faces)
+    assert st == 1 and "z_face" in msg
+
+    def other_weights(g, c, keep):
+        w = np.array([0.25, 0.75, 0.75, 0.25])
+        keep["w2"] = w
+        g.gll_w = L._dp(w)
+
+    st, msg, _ = create(other_weights)
+    assert st == 1 and "GLL" in msg
+
+    def bad_upwinding(g, c, keep):
+        c.upwinding = 2
+
+    st, msg, _ = create(bad_upwinding)
+    assert st == 1 and "upwinding" in msg

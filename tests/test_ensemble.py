@@ -106,7 +106,9 @@ def test_members_match_oracle(nsteps):
         tol = 1e-5 if nsteps == 1 else 1e-3
         for n in ("rho", "u1", "u2", "rhoe"):
             assert errs[n] <= tol, (m, n, errs[n])
-        assert errs["u3"] <= (3.0 if nsteps == 1 else 5.0) * fl, (m, errs["u3"], fl)
+        state = "MET" if errs["u3"] <= tol else "UNMET"
+        print(f"u3 gate [member {m}, {nsteps} steps]: {errs['u3']:.2e} vs stated {tol:.0e}: {state}")
+        assert errs["u3"] <= (3.0 if nsteps == 1 else 5.0) * fl, (m, errs["u3"], fl)  # regression guard
         e3.append(errs["u3"])
         f3.append(fl)
     rms_e, rms_f = float(np.sqrt(np.mean(np.square(e3)))), float(np.sqrt(np.mean(np.square(f3))))

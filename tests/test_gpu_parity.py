@@ -1,21 +1,22 @@
 """Parity of the native HIP step with the CPU restatement (the oracle), on
 the same float32-rounded seeded inputs, through the C ABI.
 
-Gates (relative L2 per field, native float32 vs oracle float64):
-  * hooks (one evaluation)             <= 1e-5  (u_3 tendency: <= 1e-4)
-  * one step, rho, u_1, u_2, rho e_tot <= 1e-5
-  * 100 column steps, the same fields  <= 1e-3
-  * u_3, against its float32 storage floor: the error of the float64 oracle
-    run with every array a float32 implementation stores (stage states,
-    tendencies, new state) rounded to float32.  u_3 starts at rest, so its
-    value after a step is a small increment of a small difference of large
-    hydrostatic terms, and storage rounding alone gives it ~1e-4 after one
-    step and ~3e-3 after 100 column steps.  The native u_3 must stay within
-    3x that floor after one step and 2x over a few steps, at or below it
-    after 100 column steps, and below half the error of the oracle built in
-    float32 throughout.  Deeper than 63 levels (the thread-per-column Newton
-    stage) the floor estimate and float32 arithmetic drift apart; there u_3
-    must stay below the float32 oracle's error.
+Stated gates (relative L2 per field, native float32 vs oracle float64), asserted
+for rho, u_1, u_2, rho e_tot:
+  * hooks (one evaluation)             <= 1e-5
+  * one step                           <= 1e-5
+  * 100 steps (a column and a sphere)  <= 1e-3
+
+u_3 against the same stated gates is REPORTED, not asserted: every test
+prints a "u3 gate" line with the value, the stated gate, MET or UNMET, and
+the float32 storage floor -- the error of the float64 oracle run with every
+array a float32 implementation stores (stage states, tendencies, new state)
+rounded to float32.  u_3 starts at rest, so after one step it is a small
+increment of a small difference of large hydrostatic terms, and storage
+rounding alone puts it at ~1e-4 after one step, above the 1e-5 gate.  The u_3
+asserts below are REGRESSION GUARDS relative to that floor (or to the error of
+the oracle built in float32 throughout), not the stated gates.
+
 At the full workload sizes the oracle is too slow for many steps, so those
 tests check size-independent properties: finiteness, mass conservation to
 float32 round-off, and bitwise continuity of duplicated element-edge columns.
@@ -52,6 +53,13 @@ def field_errors(Yg, Yo, g, interior_u3=False):
             a, b = a[1:-1], b[1:-1]
         out[name] = rel(a, b) if np.linalg.norm(b) > 0 else float(np.abs(a).max())
     return out
+
+
+def report_u3(label, value, stated, floor=None):
+    """Prints u_3 against its stated gate (reported, not asserted)."""
+    state = "MET" if value <= stated else "UNMET"
+    fl = f" storage floor {floor:.2e}" if floor is not None else ""
+    print(f"u3 gate [{label}]: {value:.2e} vs stated {stated:.0e}: {state}{fl}")
 
 
 def _setup(grid, num, Y0):
@@ -139,8 +147,11 @@ def test_hooks_match_oracle(case):
     print(case, res)
     for hook, errs in res.items():
         for name, e in errs.items():
-            tol = 1e-4 if name == "u3" else 1e-5
-            assert e <= tol, (hook, name, e)
+            if name == "u3":
+                report_u3(f"{case} {hook}", e, 1e-5)
+                assert e <= 1e-4, (hook, name, e)  # regression guard
+            else:
+                assert e <= 1e-5, (hook, name, e)
 
 
 def _float32_throughout(g, num, Y32, dt, nsteps, ref):
@@ -161,16 +172,19 @@ def test_column_one_step_and_hundred_steps():
     e1 = field_errors(got, ref, g)
     fl1 = _storage_floor(g, num, Y32, 10.0, 1, ref)
     print("1 step", e1, "u3 storage floor", fl1["u3"])
+    report_u3("column, 1 step", e1["u3"], 1e-5, fl1["u3"])
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e1[name] <= 1e-5
-    assert e1["u3"] <= 3.0 * fl1["u3"]
+    assert e1["u3"] <= 3.0 * fl1["u3"]  # regression guard
     got, ref, Y32 = _run_steps(g, num, Y0, 10.0, 100)
     e100 = field_errors(got, ref, g)
     fl = _storage_floor(g, num, Y32, 10.0, 100, ref)["u3"]
     f32_err = _float32_throughout(g, num, Y32, 10.0, 100, ref)
     print("100 steps", e100, "u3 storage floor", fl, "float32 throughout", f32_err)
+    report_u3("column, 100 steps", e100["u3"], 1e-3, fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e100[name] <= 1e-3, (name, e100[name])
+    # regression guards: at most the storage floor, at most half the float32 oracle's error
     assert e100["u3"] <= fl, (e100["u3"], fl)
     assert e100["u3"] <= 0.5 * f32_err, (e100["u3"], f32_err)
 
@@ -181,16 +195,32 @@ def test_sphere_one_and_five_steps():
     e1 = field_errors(got, ref, g)
     fl1 = _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
     print("1 step", e1, "u3 storage floor", fl1)
+    report_u3("sphere h4 z16, 1 step", e1["u3"], 1e-5, fl1)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e1[name] <= 1e-5
-    assert e1["u3"] <= 3.0 * fl1
+    assert e1["u3"] <= 3.0 * fl1  # regression guard
     got, ref, Y32 = _run_steps(g, num, Y0, 100.0, 5)
     e5 = field_errors(got, ref, g)
     fl5 = _storage_floor(g, num, Y32, 100.0, 5, ref)["u3"]
     print("5 steps", e5, "u3 storage floor", fl5)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e5[name] <= 1e-3, (name, e5[name])
-    assert e5["u3"] <= 2.0 * fl5
+    assert e5["u3"] <= 2.0 * fl5  # regression guard
+
+
+def test_sphere_hundred_steps():
+    """The stated 100-step gate on a sphere: h_elem 4, z_elem 16, dt 100 s,
+    hyperdiffusion and van Leer upwinding on, 100 steps against the float64
+    oracle."""
+    g, num, Y0 = _sphere_small()
+    got, ref, Y32 = _run_steps(g, num, Y0, 100.0, 100)
+    e = field_errors(got, ref, g)
+    fl = _storage_floor(g, num, Y32, 100.0, 100, ref)
+    print("100 sphere steps", e, "storage floor", fl)
+    report_u3("sphere h4 z16, 100 steps", e["u3"], 1e-3, fl["u3"])
+    for name in ("rho", "u1", "u2", "rhoe"):
+        assert e[name] <= 1e-3, (name, e[name])
+    assert e["u3"] <= 3.0 * fl["u3"]  # regression guard
 
 
 @pytest.mark.parametrize("dss_kind", [0, 1])
@@ -223,9 +253,10 @@ def test_small_sphere_dss_level_chunks(dss_chunk, dss_kind):
     e = field_errors(got, ref, g)
     fl = _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
     print(dss_chunk, e, "u3 storage floor", fl)
+    report_u3(f"h2 z10 dss chunk {dss_chunk} kind {dss_kind}, 1 step", e["u3"], 1e-5, fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e[name] <= 1e-5, (name, e[name])
-    assert e["u3"] <= 3.0 * fl
+    assert e["u3"] <= 3.0 * fl  # regression guard
 
 
 def _small_sphere_step_vs_oracle(z_elem, tuning):
@@ -255,10 +286,11 @@ def _small_sphere_step_vs_oracle(z_elem, tuning):
     return field_errors(got, ref, g), _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
 
 
-def _assert_one_step(e, fl):
+def _assert_one_step(e, fl, label):
+    report_u3(label, e["u3"], 1e-5, fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e[name] <= 1e-5, (name, e[name])
-    assert e["u3"] <= 3.0 * fl, (e["u3"], fl)
+    assert e["u3"] <= 3.0 * fl, (e["u3"], fl)  # regression guard
 
 
 @pytest.mark.parametrize("newton", [0, 1, 2])
@@ -270,7 +302,7 @@ def test_small_sphere_newton_schedules(newton):
 
     e, fl = _small_sphere_step_vs_oracle(10, {CA_TUNE_NEWTON: newton})
     print("newton", newton, e, "u3 storage floor", fl)
-    _assert_one_step(e, fl)
+    _assert_one_step(e, fl, f"h2 z10 newton {newton}, 1 step")
 
 
 @pytest.mark.parametrize("chunk", [0, 1, 3, 7, 10])
@@ -282,7 +314,7 @@ def test_small_sphere_explicit_chunks(chunk):
 
     e, fl = _small_sphere_step_vs_oracle(10, {CA_TUNE_EXPLICIT_CHUNK: chunk})
     print("explicit chunk", chunk, e, "u3 storage floor", fl)
-    _assert_one_step(e, fl)
+    _assert_one_step(e, fl, f"h2 z10 explicit chunk {chunk}, 1 step")
 
 
 def test_deep_sphere_takes_the_thread_per_column_newton_stage():
@@ -290,8 +322,8 @@ def test_deep_sphere_takes_the_thread_per_column_newton_stage():
     default schedule is the thread-per-column stage, the wavefront schedules
     are refused, and the step still matches the oracle.
 
-    The u_3 bound is the oracle built in float32, not a multiple of the
-    storage floor.  The floor rounds only storage to float32; the gap
+    The u_3 regression guard is the oracle built in float32, not a multiple
+    of the storage floor.  The floor rounds only storage to float32; the gap
     between it and float32 arithmetic grows with the column depth.  At
     h_elem 2 the u_3 error measured 1.1x the floor at z 30, 2.8x at z 63
     (the same for the thread-per-column and the PCR schedule) and 3.8x at
@@ -309,9 +341,10 @@ def test_deep_sphere_takes_the_thread_per_column_newton_stage():
     g, num, Y32, ref = _small_sphere_step_vs_oracle.last
     f32 = _float32_throughout(g, num, Y32, 100.0, 1, ref)
     print("z80", e, "u3 storage floor", fl, "float32 oracle", f32)
+    report_u3("h2 z80, 1 step", e["u3"], 1e-5, fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e[name] <= 1e-5, (name, e[name])
-    assert e["u3"] <= f32, (e["u3"], f32)
+    assert e["u3"] <= f32, (e["u3"], f32)  # regression guard
 
 
 def test_first_order_upwinding_and_rayleigh_sponge_column():
@@ -337,9 +370,10 @@ def test_c1_sized_sphere_one_step_parity():
     e = field_errors(got, ref, g)
     fl = _storage_floor(g, num, Y32, 100.0, 1, ref)["u3"]
     print(e, "u3 storage floor", fl)
+    report_u3("c1 h16 z63, 1 step", e["u3"], 1e-5, fl)
     for name in ("rho", "u1", "u2", "rhoe"):
         assert e[name] <= 1e-5, (name, e[name])
-    assert e["u3"] <= 3.0 * fl
+    assert e["u3"] <= 3.0 * fl  # regression guard
 
 
 def test_full_size_properties_h30():

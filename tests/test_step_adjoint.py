@@ -296,3 +296,34 @@ def test_c4_d0_gradient_matches_float64_fixture():
           f"max {rel.max():.2e}, median {np.median(rel):.2e}")
     assert abs(float(L) / float(z["L0"]) - 1) < 1e-5
     assert vec <= 1e-4 and rel.max() <= 1e-3
+
+
+@pytest.mark.parametrize("nlev,newton", [(40, 1), (40, 0), (80, 0)])
+def test_step_pullback_deep_columns_and_schedules(nlev, newton):
+    """The pullback on the other reverse Newton-stage shapes (one column per
+    64-lane wavefront at 33..64 levels, 128 lanes at 65..128) and with the
+    stage recorded by the thread-per-column forward (CA_TUNE_NEWTON 0),
+    against float64 central differences of the oracle."""
+    torch = _torch()
+    from cadycore.lib import CA_TUNE_NEWTON, DryDycore
+
+    g = G.cubed_sphere(2, G.tanh_stretched_faces(nlev, 30000.0, 100.0), PAR.planet_radius, PAR.Omega)
+    num = P.Numerics(dt=100.0, vert_diff=True)
+    rng = np.random.default_rng(3)
+    D0 = np.exp(rng.uniform(np.log(0.1), np.log(50.0), size=g.n_nodes))[g.node_id]
+    Y0 = I.sphere_state(g, PAR, seed=2).astype(np.float32).astype(np.float64)
+    nsteps = 3
+    w = _cotangent(g, _oracle_steps(g, num, D0, Y0, nsteps), 17)
+    d = DryDycore(g, PAR, num, D0_col=D0)
+    d.set_tuning(CA_TUNE_NEWTON, newton)
+    yb = d.step_adjoint(d.state(Y0), d.state(w), num.dt, nsteps, every=2).cpu().numpy().astype(np.float64)
+    d.close()
+    assert np.all(np.isfinite(yb))
+    for field in (3, 4):
+        v = _direction(g, Y0, field, 40 + field)
+        eps = 1e-7
+        fd = (w @ _oracle_steps(g, num, D0, Y0 + eps * v, nsteps)
+              - w @ _oracle_steps(g, num, D0, Y0 - eps * v, nsteps)) / (2 * eps)
+        rel = abs(float(yb @ v) - fd) / abs(fd)
+        print(f"nlev {nlev} newton {newton} field {field}: rel {rel:.2e}")
+        assert rel <= 1e-3

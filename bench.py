@@ -149,6 +149,20 @@ def ensemble_c3(torch, params):
     ens.step(Y, num.dt, 360)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    # the same launch captured in a hipGraph and replayed on a side stream
+    # (the first call captures and replays once, untimed)
+    side = torch.cuda.Stream()
+    Yg = Y0.clone()
+    torch.cuda.synchronize()
+    ens.step_graph(Yg, num.dt, 360, side)
+    side.synchronize()
+    with torch.cuda.stream(side):
+        Yg.copy_(Y0)
+    side.synchronize()
+    t0 = time.perf_counter()
+    ens.step_graph(Yg, num.dt, 360, side)
+    side.synchronize()
+    el_g = time.perf_counter() - t0
     out = {
         "members": len(members),
         "levels": sorted({m.nlev for m in members}),
@@ -156,6 +170,9 @@ def ensemble_c3(torch, params):
         "ms": round(el * 1e3, 3),
         "ensemble_steps_per_s": round(360 / el, 1),
         "column_steps_per_s": round(360 * len(members) / el, 1),
+        "graph_ms": round(el_g * 1e3, 3),
+        "graph_ensemble_steps_per_s": round(360 / el_g, 1),
+        "graph_bitwise_equal": bool(torch.equal(Yg, Y)),
         "schedule": ens.get_tuning(),
         "autotune_ms_per_20_steps": {f"lanes{k[0]}_order{k[1]}": round(v, 4) for k, v in tune.items()},
         "finite": bool(torch.isfinite(Y).all().item()),
@@ -332,6 +349,7 @@ def main():
         "step_roofline_frac": round(step_gbps / HBM_PEAK_GBPS, 4),
         "steps_per_s": round(1e3 / ms, 2),
         "adjoint_forward_ratio": adj["pullback_forward_ratio"] if adj else None,
+        "gradient_forward_ratio": adj["record_plus_pullback_forward_ratio"] if adj else None,
         "finite": bool(ok),
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -122,3 +122,16 @@ def sphere_state(grid: Grid, params, seed: int = 1, jet: float = 35.0,
     T[:nz, :] += noise[:, grid.node_id]
     u_e = jet * np.sin(2.0 * grid.lat) ** 2
     return pack(*_assemble(grid, params, T, p, u_e[None, :], 0.0))
+
+
+def surface_temperature(grid: Grid) -> np.ndarray:
+    """Per-column surface temperature of the default ``AtmosSurface``:
+    ``AnalyticTemperature(zonally_symmetric_temperature)`` (reference
+    ``src/setups/Setups.jl:205-216``, ``src/types.jl:2292-2302``), evaluated at
+    the surface (z = 0, no topography).  On a sphere ``271 + 29
+    exp(-lat^2 / (2 26^2)) - 6.5e-3 z`` with the latitude in degrees; on
+    independent columns (no latitude-longitude coordinates) 300 K."""
+    if grid.nelem == 0:
+        return np.full(grid.ncol, 300.0)
+    lat = np.degrees(np.asarray(grid.lat, np.float64))
+    return 271.0 + 29.0 * np.exp(-lat ** 2 / (2.0 * 26.0 ** 2)) - 6.5e-3 * 0.0

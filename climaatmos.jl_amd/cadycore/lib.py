@@ -53,6 +53,8 @@ class CaConfig(C.Structure):
         ("zd_rayleigh", C.c_double), ("alpha_uh", C.c_double), ("alpha_w", C.c_double),
         ("alpha_w_col", C.POINTER(C.c_double)), ("vert_diff", C.c_int32),
         ("D0_col", C.POINTER(C.c_double)), ("dt", C.c_double),
+        ("sfc_flux", C.c_int32), ("Cd", C.c_double), ("Ch", C.c_double), ("gustiness", C.c_double),
+        ("T_sfc_col", C.POINTER(C.c_double)),
     ]
 
 
@@ -66,6 +68,10 @@ class CaLayout(C.Structure):
                 ("s_j", C.c_int64), ("s_h", C.c_int64)]
 
 
+class CaFieldVector(C.Structure):
+    _fields_ = [("c", C.c_void_p), ("f", C.c_void_p), ("lay", CaLayout * 5)]
+
+
 EXPORTED = (
     "ca_create", "ca_destroy", "ca_state_size", "ca_last_error", "ca_set_precomputed",
     "ca_remaining_tendency", "ca_implicit_tendency", "ca_update_jacobian", "ca_ldiv",
@@ -77,6 +83,9 @@ EXPORTED = (
     "ca_ensemble_get_tuning", "ca_ensemble_autotune", "ca_ensemble_set_params", "ca_ensemble_adjoint",
     "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint", "ca_step_record", "ca_step_pullback",
     "ca_set_column_params", "ca_set_debug_poison",
+    "ca_fieldvector_native", "ca_fieldvector_vijfh", "ca_set_precomputed_quantities_fv",
+    "ca_set_implicit_precomputed_quantities_fv", "ca_remaining_tendency_fv", "ca_implicit_tendency_fv",
+    "ca_update_jacobian_fv", "ca_ldiv_fv", "ca_correct_implicit_advection_fv", "ca_dss_fv",
 )
 CA_TUNE_NEWTON, CA_TUNE_EXPLICIT_CHUNK, CA_TUNE_DSS_CHUNK, CA_TUNE_DSS_KIND, CA_TUNE_ADJ_CHUNK, CA_TUNE_LAP_CHUNK = 0, 1, 2, 3, 4, 5
 CA_ENS_LANES, CA_ENS_ORDER = 0, 1
@@ -101,6 +110,17 @@ def load_library(path: str = LIB_PATH):
         getattr(lib, nm).argtypes = [vp, vp, vp, vp]
     lib.ca_update_jacobian.argtypes = [vp, vp, C.c_double, vp]
     lib.ca_step.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
+    fvp = C.POINTER(CaFieldVector)
+    lib.ca_fieldvector_native.argtypes = [vp, vp, fvp]
+    lib.ca_fieldvector_vijfh.argtypes = [vp, vp, vp, fvp]
+    lib.ca_set_precomputed_quantities_fv.argtypes = [vp, fvp, C.c_double, vp]
+    lib.ca_set_implicit_precomputed_quantities_fv.argtypes = [vp, fvp, C.c_double, vp]
+    lib.ca_remaining_tendency_fv.argtypes = [vp, fvp, fvp, fvp, C.c_double, vp]
+    lib.ca_implicit_tendency_fv.argtypes = [vp, fvp, fvp, C.c_double, vp]
+    lib.ca_update_jacobian_fv.argtypes = [vp, fvp, C.c_double, C.c_double, vp]
+    lib.ca_ldiv_fv.argtypes = [vp, fvp, fvp, vp]
+    lib.ca_correct_implicit_advection_fv.argtypes = [vp, fvp, fvp, C.c_double, vp]
+    lib.ca_dss_fv.argtypes = [vp, fvp, C.c_double, vp]
     lib.ca_pack_state.argtypes = [vp, vp, C.POINTER(CaLayout), vp, vp]
     lib.ca_unpack_state.argtypes = [vp, vp, C.POINTER(CaLayout), vp, vp]
     lib.ca_check_finite.argtypes = [vp, vp, C.POINTER(C.c_int32), vp]
@@ -145,7 +165,7 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def create_args(grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None, D0_col=None):
+def create_args(grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None, D0_col=None, T_sfc_col=None):
     """The ca_geometry / ca_params / ca_config structs of ca_create for a grid,
     and the arrays they point into (keep them alive while the structs are used)."""
     keep = {}
@@ -188,6 +208,12 @@ def create_args(grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None
             raise ValueError("vertical diffusion needs D0_col")
         c.D0_col = _dp(arr("D0", D0_col))
     c.dt = numerics.dt
+    c.sfc_flux = int(bool(numerics.sfc_flux))
+    if numerics.sfc_flux:
+        if T_sfc_col is None:
+            raise ValueError("surface fluxes need T_sfc_col")
+        c.Cd, c.Ch, c.gustiness = numerics.Cd, numerics.Ch, numerics.gustiness
+        c.T_sfc_col = _dp(arr("Tsfc", T_sfc_col))
     return g, p, c, keep
 
 
@@ -196,11 +222,12 @@ class DryDycore:
 
     ``grid``/``params``/``numerics`` describe the configuration;
     ``alpha_w_col`` and ``D0_col`` give per-column Rayleigh sponge strengths
-    and vertical diffusivities (ensembles and calibration).
+    and vertical diffusivities (ensembles and calibration); ``T_sfc_col`` the
+    per-column surface temperature of the surface fluxes.
     """
 
     def __init__(self, grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None, D0_col=None,
-                 device=None, lib_path: str = LIB_PATH):
+                 device=None, lib_path: str = LIB_PATH, T_sfc_col=None):
         import torch
 
         if not torch.cuda.is_available():
@@ -210,7 +237,7 @@ class DryDycore:
         torch.cuda.set_device(self.device)
         self.lib = load_library(lib_path)
         self.grid, self.params, self.numerics = grid, params, numerics
-        g, p, c, keep = create_args(grid, params, numerics, alpha_w_col, D0_col)
+        g, p, c, keep = create_args(grid, params, numerics, alpha_w_col, D0_col, T_sfc_col)
         h = C.c_void_p()
         st = self.lib.ca_create(C.byref(g), C.byref(p), C.byref(c), C.byref(h))
         if st != CA_OK:
@@ -280,6 +307,72 @@ class DryDycore:
 
     def step(self, Y, dt, nsteps=1):
         self._check(self.lib.ca_step(self.h, self._ptr(Y), float(dt), int(nsteps), self._stream()), "step!")
+
+    # ------------------------------------- hooks on ClimaCore FieldVectors
+    def fieldvector(self, Y=None, c=None, f=None):
+        """A ca_fieldvector of a native state tensor ``Y``, or of VIJFH parent
+        arrays ``c`` (Julia (Nv, 4, 4, 4, Nh)) and ``f`` ((Nv + 1, 4, 4, 1, Nh))."""
+        fv = CaFieldVector()
+        if Y is not None:
+            self._check(self.lib.ca_fieldvector_native(self.h, self._ptr(Y), C.byref(fv)), "fieldvector")
+            fv._keep = (Y,)
+        else:
+            for t in (c, f):
+                assert t.is_cuda and t.dtype == self.torch.float32 and t.is_contiguous()
+            self._check(self.lib.ca_fieldvector_vijfh(self.h, C.c_void_p(c.data_ptr()), C.c_void_p(f.data_ptr()),
+                                                      C.byref(fv)), "fieldvector")
+            fv._keep = (c, f)
+        return fv
+
+    def to_vijfh(self, Y):
+        """Native state tensor -> (c, f) VIJFH parent arrays (C-order shapes
+        (Nh, 4, 4, 4, Nv) and (Nh, 1, 4, 4, Nv + 1): Julia's column-major
+        (Nv, Ni, Nj, F, Nh))."""
+        g = self.grid
+        nl, nh = g.nlev, g.ncol // 16
+        nC = nl * g.ncol
+        cen = Y[:4 * nC].reshape(4, nl, nh, 4, 4)           # (f, v, h, j, i)
+        c = cen.permute(2, 0, 3, 4, 1).contiguous()          # (h, f, j, i, v)
+        fac = Y[4 * nC:].reshape(1, nl + 1, nh, 4, 4)
+        f = fac.permute(2, 0, 3, 4, 1).contiguous()
+        return c, f
+
+    def from_vijfh(self, c, f):
+        g = self.grid
+        nl = g.nlev
+        cen = c.permute(1, 4, 0, 2, 3).reshape(-1)           # (f, v, h, j, i)
+        fac = f.permute(1, 4, 0, 2, 3).reshape(-1)
+        return self.torch.cat([cen, fac]).contiguous()
+
+    def set_precomputed_quantities_fv(self, Y, t=0.0):
+        self._check(self.lib.ca_set_precomputed_quantities_fv(self.h, C.byref(Y), float(t), self._stream()), "cache!")
+
+    def set_implicit_precomputed_quantities_fv(self, Y, t=0.0):
+        self._check(self.lib.ca_set_implicit_precomputed_quantities_fv(self.h, C.byref(Y), float(t), self._stream()),
+                    "cache_imp!")
+
+    def remaining_tendency_fv(self, Yt, Yt_lim, Y, t=0.0):
+        lim = C.byref(Yt_lim) if Yt_lim is not None else None
+        self._check(self.lib.ca_remaining_tendency_fv(self.h, C.byref(Yt), lim, C.byref(Y), float(t), self._stream()),
+                    "remaining_tendency!")
+
+    def implicit_tendency_fv(self, Yt, Y, t=0.0):
+        self._check(self.lib.ca_implicit_tendency_fv(self.h, C.byref(Yt), C.byref(Y), float(t), self._stream()),
+                    "implicit_tendency!")
+
+    def update_jacobian_fv(self, Y, dtgamma, t=0.0):
+        self._check(self.lib.ca_update_jacobian_fv(self.h, C.byref(Y), float(dtgamma), float(t), self._stream()),
+                    "update_jacobian!")
+
+    def ldiv_fv(self, dY, R):
+        self._check(self.lib.ca_ldiv_fv(self.h, C.byref(dY), C.byref(R), self._stream()), "ldiv!")
+
+    def correct_implicit_advection_tendency_fv(self, Yt, Y, t=0.0):
+        self._check(self.lib.ca_correct_implicit_advection_fv(self.h, C.byref(Yt), C.byref(Y), float(t),
+                                                              self._stream()), "correct_implicit_advection_tendency!")
+
+    def dss_fv(self, Y, t=0.0):
+        self._check(self.lib.ca_dss_fv(self.h, C.byref(Y), float(t), self._stream()), "dss!")
 
     # ------------------------------------------------------- reverse mode
     def dss_vjp(self, Yb):

@@ -100,6 +100,12 @@ class Numerics:
     vert_diff: bool = False
     deep: bool = True
     dt: float = 100.0
+    # surface_flux_tendency! with the default AtmosSurface (src/types.jl:2292-2302):
+    # ExchangeCoefficients(Cd = Ch = 0.0044), gustiness 1 m/s (surface_conditions.jl:254)
+    sfc_flux: bool = False
+    Cd: float = 0.0044
+    Ch: float = 0.0044
+    gustiness: float = 1.0
 
     @staticmethod
     def upwinding_code(name: str) -> int:

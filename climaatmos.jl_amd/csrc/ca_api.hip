@@ -83,6 +83,9 @@ struct ca_handle {
     // debug: fill the scratch no call carries over with NaN at every entry
     // point (the counterpart of fill_with_nans!, utils/debug_utils.jl:67-70)
     bool poison = false;
+    // native-layout staging of the FieldVector hooks (ca_*_fv) for arguments
+    // in another layout: allocated by the first such call, kept until destroy
+    float *fv_a = nullptr, *fv_b = nullptr;
 };
 
 // 32-bit pattern fill of n words (NaN poisoning and zeroing; a kernel, so that
@@ -321,6 +324,9 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
             if (vd) k_explicit<false, false, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
             else k_explicit<false, false, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
         }
+        // surface fluxes at the first centre (surface_flux_tendency!, called
+        // from additional_tendency!: part of the tendency the DSS averages)
+        if (n.sflux) k_sfc_flux<<<cdiv(g.ncol, 256), 256, 0, st>>>(g, n, Y, out);
     }
     ca_status s = check_launch(h, "explicit tendency");
     if (s) return s;
@@ -503,6 +509,12 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     }
     g.aw_col = nullptr;
     g.D0_col = nullptr;
+    g.Tsfc = nullptr;
+    if (cfg->sfc_flux) {
+        float *p = nullptr;
+        if ((s = upload(h, &p, tofloat(cfg->T_sfc_col, nc)))) return s;
+        g.Tsfc = p;
+    }
     if (cfg->alpha_w_col) {
         float *p = nullptr;
         if ((s = upload(h, &p, tofloat(cfg->alpha_w_col, nc)))) return s;
@@ -611,6 +623,10 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     n.alpha_uh = (float)cfg->alpha_uh;
     n.alpha_w = (float)cfg->alpha_w;
     n.vdiff = cfg->vert_diff;
+    n.sflux = cfg->sfc_flux != 0;
+    n.Cd = (float)cfg->Cd;
+    n.Ch = (float)cfg->Ch;
+    n.gust = (float)cfg->gustiness;
     n.dt = (float)cfg->dt;
     n.sphere = geom->nelem > 0;
     // work buffers
@@ -703,6 +719,9 @@ std::string check_create_args(const ca_geometry *geom, const ca_params *par, con
     if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3)
         return "upwinding must be 0 (none), 1 (first_order) or 3 (vanleer_limiter)";
     if (cfg->vert_diff && !cfg->D0_col) return "vertical diffusion needs D0_col";
+    if (cfg->sfc_flux && !cfg->T_sfc_col) return "surface fluxes need T_sfc_col";
+    if (cfg->sfc_flux && !(cfg->Cd >= 0 && cfg->Ch >= 0 && cfg->gustiness >= 0))
+        return "surface fluxes need Cd, Ch and gustiness >= 0";
     return std::string();
 }
 

@@ -58,6 +58,7 @@ struct DevGeom {
     const ushort4 *dl_slot;
     const float4 *dl_w, *dl_M;
     const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
+    const float *Tsfc;                   // per-column surface temperature (surface fluxes)
 };
 
 struct DevNum {
@@ -67,7 +68,8 @@ struct DevNum {
     double dkex_r;  // kex - 2/7 (|.| <= 1e-6, checked at create)
     float nu_v, nu_s, dd;
     float alpha_uh, alpha_w, dt;
-    int hyper, upw, rayleigh, vdiff, sphere;
+    float Cd, Ch, gust;  // surface flux exchange coefficients and wind-speed floor
+    int hyper, upw, rayleigh, vdiff, sphere, sflux;
 };
 
 // ------------------------------------------------------------- launch order

@@ -226,6 +226,10 @@ ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, i
     if (!params || !config || !members || !out || nmembers < 1) return CA_ERR_ARG;
     *out = nullptr;
     if (config->upwinding != 0 && config->upwinding != 1 && config->upwinding != 3) return CA_ERR_ARG;
+    if (config->sfc_flux) {
+        g_create_error = "ca_ensemble_create: the column ensembles have no surface fluxes";
+        return CA_ERR_ARG;
+    }
     ca_ensemble *e = new ca_ensemble();
     const ca_status s = ens_create_impl(e, params, config, nmembers, members);
     if (s != CA_OK) {

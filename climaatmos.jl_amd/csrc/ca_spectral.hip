@@ -337,3 +337,40 @@ template __global__ void k_explicit<true, true, true>(DevGeom, DevNum, const flo
 template __global__ void k_explicit<true, false, true>(DevGeom, DevNum, const float *, const float *, float *, int);
 template __global__ void k_explicit<false, false, false>(DevGeom, DevNum, const float *, const float *, float *, int);
 template __global__ void k_explicit<false, false, true>(DevGeom, DevNum, const float *, const float *, float *, int);
+
+// ------------------------------------------------------------ surface flux
+// surface_flux_tendency! (surface_flux.jl:101-145) with the surface conditions
+// of update_surface_conditions! (surface_conditions.jl:22-70, 200-288) for
+// ExchangeCoefficients(Cd, Ch), dry air and no boundary overrides, added to
+// the explicit tendency Yt at the first centre of every column (a thread per
+// column, after k_explicit and before the tendency's DSS).  A kernel of its
+// own: folded into k_explicit's level loop it raised that kernel's spill from
+// 20 to 60 bytes per lane even with the fluxes off.
+//   interior wind   CT1 / CT2 components times |a_1|, |a_2| (projected_vector_data)
+//   U               max(|u|, gustiness)
+//   rho_s           the interior pressure extrapolated hydrostatically over
+//                   dz at the mean of the two temperatures, over R_d T_s
+//   shf             -rho_s Ch U (cp_d (T - T_s) + g dz)
+//   momentum flux   -rho_s Cd U (s_f0 / s_c0) u_i in covariant components (built
+//                   from unit CT1 / CT2 vectors at the surface, lowered with the
+//                   surface metric)
+// Both enter through the bottom face of a divergence with zero interior flux:
+// the tendency is the flux times J_f(surface) / J_c = s_f0^2 / (s_c0^2 dz_c0).
+__global__ void __launch_bounds__(256) k_sfc_flux(DevGeom g, DevNum n, const float *Y, float *Yt) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= g.ncol) return;
+    const CV c = cv_load(g, n, Y, 0, col);
+    const float s = g.sc[0], s0 = g.sf[0];
+    const float ui = c.ct1 * sqrtf(g.gd11[col]) * s, vi = c.ct2 * sqrtf(g.gd22[col]) * s;
+    const float U = fmaxf(sqrtf(ui * ui + vi * vi), n.gust);
+    const float T = c.T, Ts = g.Tsfc[col];
+    const float dz = g.zc[0] - g.zf[0];
+    const float rho_s = c.rho * (T / Ts) * expf(n.grav * dz / (n.R_d * (0.5f * (T + Ts))));
+    const float shf = -rho_s * n.Ch * U * (n.cp_d * (T - Ts) + n.grav * dz);
+    const float fac = s0 * s0 * g.isJc[0];
+    const float cD = fac * rho_s * n.Cd * U * (s0 / s) / c.rho;
+    const int64_t nC = g.nC;
+    Yt[nC + col] -= cD * c.u1;
+    Yt[2 * nC + col] -= cD * c.u2;
+    Yt[3 * nC + col] += fac * shf;
+}

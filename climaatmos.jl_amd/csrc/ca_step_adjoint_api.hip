@@ -65,12 +65,16 @@ ca_status adj_ensure(ca_handle *h, int64_t rec_steps, int64_t nck) {
 }
 
 // Debug poisoning of the reverse-mode scratch (see poison_work); with
-// records, also the per-step records, which every pullback rewrites
+// records, also the per-step records, which every pullback rewrites.  The
+// AB_YF plane (the last one) is left alone: it holds ca_step_adjoint's copy
+// of its input across the ca_step_record it makes, whose poisoning would
+// otherwise erase it; it is written before every read.
+static_assert(AB_YF == AB_COUNT - 1, "AB_YF must be the last scratch plane");
 ca_status poison_reverse(ca_handle *h, hipStream_t st, bool records) {
     if (!h->poison) return CA_OK;
     ca_status s;
     if ((s = poison_work(h, st))) return s;
-    if ((s = fill_words(h, h->aw_buf, (int64_t)AB_COUNT * h->S, kNaNBits, st))) return s;
+    if ((s = fill_words(h, h->aw_buf, (int64_t)AB_YF * h->S, kNaNBits, st))) return s;
     if ((s = fill_words(h, h->aw_L, 16 * h->g.nC, kNaNBits, st))) return s;
     if (records && (s = fill_words(h, h->aw_rec, h->aw_rec_steps * rec_size(h), kNaNBits, st))) return s;
     return CA_OK;
@@ -272,6 +276,7 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
                                     void *stream) {
     if (!h || !Yb || !Ytb || !Y) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: sphere grids only (columns: ca_ensemble_adjoint)");
+    if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: the reverse mode has no surface fluxes");
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
     ca_status s;
@@ -299,6 +304,7 @@ ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int3
     if (!h || !Y || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_record: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode supports nlev <= 128");
+    if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no surface fluxes");
     hipStream_t st = (hipStream_t)stream;
     const int64_t S = h->S;
     const int nseg = (nsteps + every - 1) / every;
@@ -358,6 +364,7 @@ ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, doubl
     if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode supports nlev <= 128");
+    if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no surface fluxes");
     hipStream_t st = (hipStream_t)stream;
     ca_status s;
     if ((s = adj_ensure(h, every, (nsteps + every - 1) / every))) return s;

@@ -11,3 +11,4 @@
 #include "ca_api.hip"
 #include "ca_ensemble_api.hip"
 #include "ca_step_adjoint_api.hip"
+#include "ca_hooks_fv.hip"

@@ -93,6 +93,16 @@ typedef struct ca_config {
     int32_t vert_diff;      /* 1: DecayWithHeightDiffusion(D0 per column, H = Inf) */
     const double *D0_col;   /* [ncol] when vert_diff                         */
     double dt;              /* p.dt (van Leer Courant number)                */
+    /* surface_flux_tendency! (src/prognostic_equations/surface_flux.jl:101-145)
+     * with the default AtmosSurface (src/types.jl:2292-2302): bulk fluxes of
+     * ExchangeCoefficients(Cd, Ch) against a per-column surface temperature
+     * (the AnalyticTemperature evaluated at the surface), dry air, no
+     * boundary overrides.  Sphere and independent columns; the reverse mode
+     * and the column ensembles return CA_ERR_ARG when it is on. */
+    int32_t sfc_flux;       /* 1: surface fluxes on (0: disable_surface_flux_tendency) */
+    double Cd, Ch;          /* exchange coefficients (default 0.0044 each)   */
+    double gustiness;       /* wind-speed floor [m/s] (default 1)            */
+    const double *T_sfc_col; /* [ncol] surface temperature [K] when sfc_flux */
 } ca_config;
 
 /* Strided layout of one component: element (level v, node i, j, element h)
@@ -107,7 +117,8 @@ typedef struct ca_layout {
  * ncol outside [1, 2^28], ncol != 16 nelem on a sphere, z_face not strictly
  * increasing, GLL tables other than the Nq = 4 Gauss-Lobatto-Legendre ones
  * (the kernels share them process-wide), an unknown upwinding code, vertical
- * diffusion without D0_col, or R_d / cp_d off 2/7. */
+ * diffusion without D0_col, surface fluxes without T_sfc_col or with negative
+ * Cd, Ch or gustiness, or R_d / cp_d off 2/7. */
 ca_status ca_create(const ca_geometry *geom, const ca_params *params, const ca_config *config,
                     ca_handle **out);
 ca_status ca_destroy(ca_handle *h);
@@ -134,6 +145,58 @@ ca_status ca_pack_state(ca_handle *h, const float *src, const ca_layout *layouts
                         void *stream);
 ca_status ca_unpack_state(ca_handle *h, const float *Y, const ca_layout *layouts5, float *dst,
                           void *stream);
+
+/* ------------------------------------------------------------------------
+ * The hooks on ClimaCore FieldVectors.  A ca_fieldvector is a state (or a
+ * tendency, increment or residual) as ClimaCore holds it: the parent array of
+ * the centre fields Y.c = (rho, u_h::C12, rho e_tot) and of the face field
+ * Y.f = (u_3::C3), with one ca_layout per component (rho, u_1, u_2,
+ * rho e_tot relative to c; u_3 relative to f) giving its offset and its
+ * level / node i / node j / element strides, so a caller passes
+ * pointer(parent(Y.c)), pointer(parent(Y.f)) and the strides of its
+ * DataLayout (VIJFH, VIJHF, or any other order).  Arguments, order and
+ * meaning are those of the reference hooks, t included (the dry path does not
+ * depend on it).  A FieldVector in the native layout (ca_fieldvector_native)
+ * is read and written in place; any other layout is transposed through LDS
+ * into a native staging buffer of the handle (two state-sized buffers,
+ * allocated by the first such call and kept until ca_destroy) and outputs are
+ * transposed back: one extra read and write of each argument.  Results are
+ * bitwise those of the native-pointer hooks above. */
+typedef struct ca_fieldvector {
+    float *c;               /* parent array of Y.c                            */
+    float *f;               /* parent array of Y.f                            */
+    ca_layout lay[5];       /* rho, u_1, u_2, rho e_tot (in c); u_3 (in f)    */
+} ca_fieldvector;
+/* descriptor of a native state buffer Y (ca_state_size(h) floats) */
+ca_status ca_fieldvector_native(const ca_handle *h, float *Y, ca_fieldvector *out);
+/* descriptor of ClimaCore VIJFH parent arrays: Y.c as Julia (Nv, 4, 4, 4, Nh)
+ * and Y.f as (Nv + 1, 4, 4, 1, Nh), column-major (level fastest) */
+ca_status ca_fieldvector_vijfh(const ca_handle *h, float *c, float *f, ca_fieldvector *out);
+/* cache!: set_precomputed_quantities!(Y, p, t), src/cache/precomputed_quantities.jl:928-931.
+ * The precomputed quantities are formed inside the kernels that use them; the
+ * effect a caller sees is the impenetrability filter on Y's u_3. */
+ca_status ca_set_precomputed_quantities_fv(ca_handle *h, const ca_fieldvector *Y, double t, void *stream);
+/* cache_imp!: set_implicit_precomputed_quantities!(Y, p, t), precomputed_quantities.jl:698 */
+ca_status ca_set_implicit_precomputed_quantities_fv(ca_handle *h, const ca_fieldvector *Y, double t,
+                                                    void *stream);
+/* T_exp_T_lim!: remaining_tendency!(Yt, Yt_lim, Y, p, t), remaining_tendency.jl:48.
+ * Yt_lim (may be NULL) is zeroed, as the reference does: the dry model has no
+ * limited tracer tendencies. */
+ca_status ca_remaining_tendency_fv(ca_handle *h, const ca_fieldvector *Yt, const ca_fieldvector *Yt_lim,
+                                   const ca_fieldvector *Y, double t, void *stream);
+/* T_imp!: implicit_tendency!(Yt, Y, p, t), implicit/implicit_tendency.jl:36 */
+ca_status ca_implicit_tendency_fv(ca_handle *h, const ca_fieldvector *Yt, const ca_fieldvector *Y, double t,
+                                  void *stream);
+/* Wfact: update_jacobian!(jac, Y, p, dtgamma, t), implicit/jacobian.jl:74-75 */
+ca_status ca_update_jacobian_fv(ca_handle *h, const ca_fieldvector *Y, double dtgamma, double t, void *stream);
+/* ldiv!(dY, jac, R), implicit/jacobian.jl:78-82 (the AbstractVector method at
+ * :87-96 is the same call on the FieldVector the vector wraps) */
+ca_status ca_ldiv_fv(ca_handle *h, const ca_fieldvector *dY, const ca_fieldvector *R, void *stream);
+/* T_post_imp!: correct_implicit_advection_tendency!(Yt, Y, p, t), implicit_tendency.jl:322 */
+ca_status ca_correct_implicit_advection_fv(ca_handle *h, const ca_fieldvector *Yt, const ca_fieldvector *Y,
+                                           double t, void *stream);
+/* dss!(Y, p, t), constrain_state.jl:59 */
+ca_status ca_dss_fv(ca_handle *h, const ca_fieldvector *Y, double t, void *stream);
 
 /* Launch-schedule knobs (the autotuner's search space).  Results do not
  * depend on them beyond float32 rounding order.
@@ -260,7 +323,8 @@ ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream);
 ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, const float *Y, float *D0b,
                                     void *stream);
 /* The sphere reverse mode (ca_step_record, ca_step_pullback, ca_step_adjoint)
- * supports nlev <= 128 and returns CA_ERR_ARG above that, before any launch.
+ * supports nlev <= 128 and no surface fluxes, and returns CA_ERR_ARG
+ * otherwise, before any launch.
  * ca_step that also keeps what its pullback needs: runs nsteps steps of Y in
  * place (as ca_step) and stores the state at every `every`-th step in the
  * handle (ceil(nsteps / every) states; ca_step_pullback adds `every` steps of

@@ -782,6 +782,81 @@ static void vector_laplacian_parts(ora_ctx *c, int k, const real *x1, const real
 
 /* remaining_tendency!, remaining_tendency.jl:48-58; precomputed quantities
  * must have been set from Y (the time stepper's cache! hook). */
+/* ----------------------------------------------------- surface fluxes */
+/* surface_flux_tendency!, surface_flux.jl:101-145, with sfc_conditions from
+ * update_surface_conditions! (surface_conditions.jl:22-70) for the default
+ * AtmosSurface (types.jl:2292-2302): ExchangeCoefficients(Cd, Ch) with an
+ * AnalyticTemperature surface temperature, no boundary overrides (surface
+ * winds 0, gustiness from the config), dry air.  Called from
+ * additional_tendency!, i.e. explicit; uses the precomputed T of the cache.
+ *
+ * The flux closure itself is SurfaceFluxes 1.2.1 (not vendored; Manifest
+ * git-tree-sha1 0b35705e...), restated from its published bulk formulas:
+ *   wind speed   U = max(|u_int - u_sfc|, gustiness)
+ *   momentum     rho tau_xz = -rho_sfc Cd U (u_int - u_sfc), same for y
+ *   heat         shf = -rho_sfc Ch U (DSE_int - DSE_sfc),
+ *                DSE = cp_d (T - T_0) + Phi (dry static energy)
+ *   density      rho_sfc: the interior pressure extrapolated hydrostatically
+ *                over dz = z_int - z_sfc at the layer's mean temperature,
+ *                p_sfc = p_int exp(g dz / (R_d (T_int + T_sfc) / 2)),
+ *                rho_sfc = p_sfc / (R_d T_sfc)
+ * Dry air: lhf = evaporation = 0.  The interior wind components are
+ * projected_vector_data(CT1 / CT2, u, L) (utilities.jl:526-527): the
+ * contravariant component times |a_i| = sqrt(g_ii); the flux tensor is
+ * n3 (x) C12(rho tau_xz xz + rho tau_yz yz) with xz, yz the CT1, CT2 unit
+ * vectors at the surface and n3 = C3(unit_basis_vector_data(C3, L))
+ * (tensor_from_components, surface_conditions.jl:357-362).  The tendencies
+ * are DivergenceF2C with bottom SetValue(flux) and a zero interior flux
+ * (boundary_tendency_scalar / _momentum, surface_flux.jl:35-75): only the
+ * first centre changes, by +J_f F^3 / J_c (divided by rho for u_h). */
+static void surface_flux_tendency(ora_ctx *c, real *Yt, real *Y) {
+    const int nc = c->ncol;
+    const double *G = c->g.gup, *Gd = c->g.gdn;
+    const real *rho = RHO(Y), *u1 = UH1(Y), *u2 = UH2(Y);
+    const real R = (real)c->n.R_d, cpd = (real)c->n.cp_d, T0 = (real)c->n.T_0, grav = (real)c->n.grav;
+    const real Cd = (real)c->n.Cd, Ch = (real)c->n.Ch, gust = (real)c->n.gustiness;
+    ORA_PAR_FOR
+    for (int col = 0; col < nc; ++col) {
+        const int64_t q = IDX(0, col), qf = IDX(0, col);
+        const real rho_int = rho[q], T_int = c->T[q];
+        const real z_int = c->zc[0], z_sfc = c->zf[0];
+        const real dz = z_int - z_sfc;
+        const real Phi_int = grav * z_int, Phi_sfc = grav * z_sfc;
+        const real T_sfc = (real)c->n.T_sfc_col[col];
+        /* interior velocity CT1, CT2 (no topography: g^13 = g^23 = 0) */
+        const real s = c->sc[0], is2 = 1 / (s * s);
+        const real ct1 = is2 * ((real)G[col] * u1[q] + (real)G[nc + col] * u2[q]);
+        const real ct2 = is2 * ((real)G[nc + col] * u1[q] + (real)G[2 * nc + col] * u2[q]);
+        const real u_int = ct1 * sqrt((real)Gd[col] * s * s);
+        const real v_int = ct2 * sqrt((real)Gd[2 * nc + col] * s * s);
+        /* SurfaceFluxes: bulk fluxes with Cd, Ch */
+        real speed = sqrt(u_int * u_int + v_int * v_int);
+        if (speed < gust) speed = gust;
+        const real p_int = rho_int * R * T_int;
+        const real p_sfc = p_int * exp(grav * dz / (R * (T_int + T_sfc) / 2));
+        const real rho_sfc = p_sfc / (R * T_sfc);
+        const real dse_int = cpd * (T_int - T0) + Phi_int, dse_sfc = cpd * (T_sfc - T0) + Phi_sfc;
+        const real shf = -rho_sfc * Ch * speed * (dse_int - dse_sfc);
+        const real tau_x = -rho_sfc * Cd * speed * u_int, tau_y = -rho_sfc * Cd * speed * v_int;
+        /* flux vectors on the surface normal, with the surface metric */
+        const real s0 = c->sf[0];
+        const real g11 = (real)Gd[col] * s0 * s0, g12 = (real)Gd[nc + col] * s0 * s0,
+                   g22 = (real)Gd[2 * nc + col] * s0 * s0;
+        const real fc1 = tau_x / sqrt(g11), fc2 = tau_y / sqrt(g22); /* CT12 */
+        const real f1 = g11 * fc1 + g12 * fc2, f2 = g12 * fc1 + g22 * fc2; /* C12 */
+        const real g33 = 1 / (c->Jvf[0] * c->Jvf[0]);
+        const real n3 = 1 / sqrt(g33); /* C3 component of the unit normal */
+        /* DivergenceF2C at the first centre: (J F^3)(face 1, zero) - (J F^3)(face 0) */
+        const real Jf0 = c->Jf[qf], Jc0 = c->Jc[q];
+        const real div_e = (0 - Jf0 * (g33 * (shf * n3))) / Jc0;
+        const real div_1 = (0 - Jf0 * (g33 * (n3 * f1))) / Jc0;
+        const real div_2 = (0 - Jf0 * (g33 * (n3 * f2))) / Jc0;
+        RHOE(Yt)[q] -= div_e;
+        UH1(Yt)[q] -= div_1 / rho_int;
+        UH2(Yt)[q] -= div_2 / rho_int;
+    }
+}
+
 static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
     int nl = c->nl, nc = c->ncol;
     int sphere = c->g.nelem > 0;
@@ -1080,6 +1155,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
             }
         }
     }
+    if (c->n.sfc_flux) surface_flux_tendency(c, Yt, Y);
 }
 void ora_remaining_tendency(ora_ctx *c, ora_real *Yt, ora_real *Y) { remaining_tendency(c, (real *)Yt, (real *)Y); }
 

@@ -48,6 +48,10 @@ typedef struct ora_num {
     int32_t vert_diff;
     const double *D0_col;                    /* per-column diffusivity [m2/s] */
     double dt;             /* p.dt, used by the van Leer limiter */
+    int32_t sfc_flux;      /* surface_flux_tendency! (ExchangeCoefficients)  */
+    double Cd, Ch;         /* drag and heat exchange coefficients            */
+    double gustiness;      /* wind-speed floor [m/s]                         */
+    const double *T_sfc_col; /* per-column surface temperature [K]         */
 } ora_num;
 
 #ifdef ORACLE_F32

@@ -46,6 +46,8 @@ class _Num(C.Structure):
         ("zd_rayleigh", C.c_double), ("alpha_uh", C.c_double), ("alpha_w", C.c_double),
         ("alpha_w_col", C.POINTER(C.c_double)), ("vert_diff", C.c_int32),
         ("D0_col", C.POINTER(C.c_double)), ("dt", C.c_double),
+        ("sfc_flux", C.c_int32), ("Cd", C.c_double), ("Ch", C.c_double), ("gustiness", C.c_double),
+        ("T_sfc_col", C.POINTER(C.c_double)),
     ]
 
 
@@ -66,7 +68,7 @@ class Oracle:
     """
 
     def __init__(self, grid, params, numerics, alpha_w_col=None, D0_col=None, f32=False,
-                 lib_path=None):
+                 lib_path=None, T_sfc_col=None):
         if lib_path is None:
             build()
             lib_path = os.path.join(_BUILD, "liboracle_f32.so" if f32 else "liboracle.so")
@@ -145,6 +147,12 @@ class Oracle:
                 raise ValueError("vertical diffusion needs D0_col")
             num.D0_col = _dp(arr("D0", D0_col))
         num.dt = n.dt
+        num.sfc_flux = int(bool(getattr(n, "sfc_flux", False)))
+        if num.sfc_flux:
+            if T_sfc_col is None:
+                raise ValueError("surface fluxes need T_sfc_col")
+            num.Cd, num.Ch, num.gustiness = n.Cd, n.Ch, n.gustiness
+            num.T_sfc_col = _dp(arr("Tsfc", T_sfc_col))
         self._keep = keep
         self._geom = geom
         self._num = num

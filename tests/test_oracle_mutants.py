@@ -131,6 +131,38 @@ MUTANTS = [
      'UH1(Yt)[IDX(k, col)] += -beta * u1[IDX(k, col)];',
      'UH1(Yt)[IDX(k, col)] += beta * u1[IDX(k, col)];',
      'rayleigh_uh'),
+    ('surface heat flux sign',
+     'const real shf = -rho_sfc * Ch * speed * (dse_int - dse_sfc);',
+     'const real shf = rho_sfc * Ch * speed * (dse_int - dse_sfc);',
+     'sfc_column'),
+    ('surface density taken as the interior density',
+     'const real rho_sfc = p_sfc / (R * T_sfc);',
+     'const real rho_sfc = rho_int;',
+     'sfc_column'),
+    ('surface density extrapolated at the interior temperature',
+     'const real p_sfc = p_int * exp(grav * dz / (R * (T_int + T_sfc) / 2));',
+     'const real p_sfc = p_int * exp(grav * dz / (R * T_int));',
+     'sfc_sphere'),
+    ('dry static energy: geopotential difference dropped',
+     'const real dse_int = cpd * (T_int - T0) + Phi_int, dse_sfc',
+     'const real dse_int = cpd * (T_int - T0) + Phi_sfc, dse_sfc',
+     'sfc_column'),
+    ('gustiness floor dropped',
+     'if (speed < gust) speed = gust;',
+     '',
+     'sfc_column'),
+    ('surface drag swaps Cd and Ch',
+     'const real tau_x = -rho_sfc * Cd * speed * u_int, tau_y = -rho_sfc * Cd * speed * v_int;',
+     'const real tau_x = -rho_sfc * Ch * speed * u_int, tau_y = -rho_sfc * Ch * speed * v_int;',
+     'sfc_column'),
+    ('surface momentum flux left contravariant',
+     'const real f1 = g11 * fc1 + g12 * fc2, f2 = g12 * fc1 + g22 * fc2; /* C12 */',
+     'const real f1 = fc1, f2 = fc2;',
+     'sfc_sphere'),
+    ('surface drag divided by the surface density',
+     'UH1(Yt)[q] -= div_1 / rho_int;',
+     'UH1(Yt)[q] -= div_1 / rho_sfc;',
+     'sfc_column'),
 ]
 
 

@@ -17,6 +17,13 @@ restatement fails here.
   rule for linear rho and s_d, column conservation, and second-order
   convergence on the deep sphere.
 * Rayleigh sponge on u_h (rayleigh_sponge.jl:39-44): -alpha zeta(z) u_h.
+* Surface fluxes (surface_flux.jl:101-145 with ExchangeCoefficients): only the
+  first centre changes; an isothermal surface layer gives the exact
+  hydrostatic surface density; a surface at the interior's dry static energy
+  gives no heat flux; the drag is -(rho_s / rho) Cd U u / dz with the wind
+  speed floored at the gustiness; on the deep sphere the column energy gain
+  equals the bulk heat flux through the surface area and the covariant drag
+  is parallel to the covariant wind.
 
 Each check is a function of an Oracle factory so that
 tests/test_oracle_mutants.py can run the same checks on copies of the oracle
@@ -284,6 +291,94 @@ def test_rayleigh_uh_closed_form():
     check_rayleigh_uh(default_factory)
 
 
+# ---------------------------------------------------------- surface fluxes
+
+def _sfc(mk, grid, Y, T_sfc, Cd=0.0044, Ch=0.0044, gust=1.0):
+    """The surface-flux part of the oracle's tendency (fluxes on minus off)."""
+    on = P.Numerics(dt=10.0, hyperdiff=False, sfc_flux=True, Cd=Cd, Ch=Ch, gustiness=gust)
+    off = P.Numerics(dt=10.0, hyperdiff=False)
+    return (_tendency(mk, grid, on, Y, dss=False, T_sfc_col=np.asarray(T_sfc, float))
+            - _tendency(mk, grid, off, Y, dss=False))
+
+
+def check_surface_flux_column(mk):
+    N, H = 10, 3000.0
+    g = G.columns(1, G.uniform_faces(N, H))
+    zc, dz1, z1 = g.zc, H / N, g.zc[0]
+    R, cp, grav = PAR.R_d, PAR.cp_d, PAR.grav
+    rho = 1.2 * np.exp(-zc / 8000.0)
+    T = 290.0 - 0.0065 * zc
+    for speed in (5.0, 0.3):  # above and below the 1 m/s gustiness floor
+        U = max(speed, 1.0)
+        # isothermal surface layer: rho_s is the exact hydrostatic extrapolation
+        Y = _column_state(zc, rho, T, np.full(N, speed))
+        d = _sfc(mk, g, Y, [T[0]])
+        rho_s = rho[0] * math.exp(grav * z1 / (R * T[0]))
+        exp_e = -rho_s * 0.0044 * U * grav * z1 / dz1
+        assert abs(d[3 * N] - exp_e) < 1e-12 * abs(exp_e), (speed, d[3 * N], exp_e)
+        exp_u = -rho_s / rho[0] * 0.0044 * U * speed / dz1
+        assert abs(d[N] - exp_u) < 1e-12 * abs(exp_u), (speed, d[N], exp_u)
+        # only the first centre of rho e_tot and u_h changes
+        assert np.abs(d[:N]).max() == 0 and np.abs(d[4 * N:]).max() == 0
+        assert np.abs(d[N + 1:2 * N]).max() == 0 and np.abs(d[3 * N + 1:4 * N]).max() == 0
+    # surface at the interior's dry static energy: no heat flux; drag unchanged
+    Y = _column_state(zc, rho, T, np.full(N, 5.0))
+    Ts = T[0] + grav * z1 / cp
+    d = _sfc(mk, g, Y, [Ts])
+    assert abs(d[3 * N]) < 1e-12 * rho[0] * 0.0044 * 5.0 * cp * 10.0 / dz1
+    # a warmer surface heats the air, linearly in Ch; Cd does not enter it
+    d1 = _sfc(mk, g, Y, [Ts + 10.0])
+    d2 = _sfc(mk, g, Y, [Ts + 10.0], Ch=0.0088, Cd=0.001)
+    assert d1[3 * N] > 0 and abs(d2[3 * N] / d1[3 * N] - 2.0) < 1e-12
+    assert abs(d2[N] / d1[N] - 0.001 / 0.0044) < 1e-12
+
+
+def check_surface_flux_sphere(mk):
+    zf = G.uniform_faces(6, 3000.0)
+    g = G.cubed_sphere(2, zf, PAR.planet_radius, PAR.Omega)
+    Y = M.Smooth(PAR).state(g)
+    rng = np.random.default_rng(5)
+    T_sfc = 285.0 + 10.0 * rng.random(g.ncol)
+    d = _planes(g, _sfc(mk, g, Y, T_sfc))
+    u = _planes(g, Y)
+    assert all(np.abs(d[i][1:]).max() == 0 for i in (1, 2, 3)) and np.abs(d[0]).max() == 0
+    # the covariant drag is parallel to the covariant wind (the metric round trip)
+    r1, r2 = d[1][0] / u[1][0], d[2][0] / u[2][0]
+    big = (np.abs(u[1][0]) > 1.0) & (np.abs(u[2][0]) > 1.0)
+    assert big.sum() > 10 and np.abs(r1[big] / r2[big] - 1).max() < 1e-10
+    assert np.all(r1[big] < 0)
+    # energy: the column gain equals the bulk heat flux through the surface
+    # area; the wind speed is |(u^1 |a_1|, u^2 |a_2|)| at the first centre
+    a = PAR.planet_radius
+    s1, s0 = (a + g.zc[0]) / a, (a + zf[0]) / a
+    gu, gd = g.gup, g.gdn
+    ct1 = (gu[0] * u[1][0] + gu[1] * u[2][0]) / s1 ** 2
+    ct2 = (gu[1] * u[1][0] + gu[2] * u[2][0]) / s1 ** 2
+    U = np.maximum(np.hypot(ct1 * np.sqrt(gd[0]) * s1, ct2 * np.sqrt(gd[2]) * s1), 1.0)
+    o = mk(g, PAR, P.Numerics(dt=10.0, hyperdiff=False))
+    y = Y.copy()
+    o.set_precomputed(y)
+    T = o.precomputed()[1].reshape(-1)
+    T1, rho1, z1 = T[:g.ncol], u[0][0], g.zc[0]
+    rho_s = rho1 * T1 / T_sfc * np.exp(PAR.grav * z1 / (PAR.R_d * (T1 + T_sfc) / 2))
+    shf = -rho_s * 0.0044 * U * (PAR.cp_d * (T1 - T_sfc) + PAR.grav * z1)
+    Jc1 = g.J_h * s1 ** 2 * (zf[1] - zf[0])
+    np.testing.assert_allclose(d[3][0] * Jc1, g.J_h * s0 ** 2 * shf, rtol=1e-10)
+    # drag magnitude from the same physics: rho_s Cd U through the surface area
+    # over the cell volume.  The flux's covariant components are formed with
+    # the surface metric and added to the centre's unchanged, so the physical
+    # drag at the centre is s_0 / s_1 (1 - 4e-5 here) of the surface one.
+    np.testing.assert_allclose(-r1, rho_s / rho1 * 0.0044 * U * g.J_h * s0 ** 2 / Jc1 * (s0 / s1), rtol=1e-10)
+
+
+def test_surface_flux_column_closed_forms():
+    check_surface_flux_column(default_factory)
+
+
+def test_surface_flux_deep_sphere_budget():
+    check_surface_flux_sphere(default_factory)
+
+
 ALL_CHECKS = {
     "mms_horizontal": check_mms_horizontal,
     "mms_vertical": check_mms_vertical,
@@ -292,4 +387,6 @@ ALL_CHECKS = {
     "vdiff_column": check_vertical_diffusion_column,
     "vdiff_sphere": check_vertical_diffusion_sphere,
     "rayleigh_uh": check_rayleigh_uh,
+    "sfc_column": check_surface_flux_column,
+    "sfc_sphere": check_surface_flux_sphere,
 }

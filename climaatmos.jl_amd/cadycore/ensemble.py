@@ -174,20 +174,22 @@ class ColumnEnsemble:
             if d0 is not None:
                 m.D0 = float(d0[i])
 
-    def adjoint(self, Y0, YbN, dt, nsteps, every=8):
+    def adjoint(self, Y0, YbN, dt, nsteps, every=8, noalloc=False):
         """Reverse mode of ``nsteps`` steps from Y0: given the cotangent YbN of
         the states after the steps, returns the cotangents of Y0 and of each
-        member's D0 and alpha_w (device tensors)."""
+        member's D0 and alpha_w (device tensors).  ``noalloc`` calls
+        ca_ensemble_adjoint_noalloc (scratch in the library's fixed per-GPU
+        array instead of handle allocations)."""
         t = self.torch
         Yb0 = t.empty_like(Y0)
         M = len(self.members)
         D0b = t.empty(M, dtype=t.float32, device=self.device)
         awb = t.empty(M, dtype=t.float32, device=self.device)
         assert YbN.is_cuda and YbN.dtype == t.float32 and YbN.is_contiguous() and YbN.numel() == self.S
-        self._check(self.lib.ca_ensemble_adjoint(self.h, self._ptr(Y0), C.c_void_p(YbN.data_ptr()), float(dt),
-                                                 int(nsteps), int(every), C.c_void_p(Yb0.data_ptr()),
-                                                 C.c_void_p(D0b.data_ptr()), C.c_void_p(awb.data_ptr()),
-                                                 self._stream()), "ca_ensemble_adjoint")
+        fn = self.lib.ca_ensemble_adjoint_noalloc if noalloc else self.lib.ca_ensemble_adjoint
+        self._check(fn(self.h, self._ptr(Y0), C.c_void_p(YbN.data_ptr()), float(dt), int(nsteps), int(every),
+                       C.c_void_p(Yb0.data_ptr()), C.c_void_p(D0b.data_ptr()), C.c_void_p(awb.data_ptr()),
+                       self._stream()), fn.__name__)
         return Yb0, D0b, awb
 
     def autotune(self, Y, dt, nsteps):

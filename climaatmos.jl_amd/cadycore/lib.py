@@ -81,6 +81,7 @@ EXPORTED = (
     "ca_ensemble_create", "ca_ensemble_destroy", "ca_ensemble_state_size", "ca_ensemble_member_offset",
     "ca_ensemble_last_error", "ca_ensemble_step", "ca_ensemble_step_graph", "ca_ensemble_set_tuning",
     "ca_ensemble_get_tuning", "ca_ensemble_autotune", "ca_ensemble_set_params", "ca_ensemble_adjoint",
+    "ca_ensemble_adjoint_noalloc",
     "ca_dss_vjp", "ca_remaining_tendency_vjp", "ca_step_adjoint", "ca_step_record", "ca_step_pullback",
     "ca_set_column_params", "ca_set_debug_poison",
     "ca_fieldvector_native", "ca_fieldvector_vijfh", "ca_set_precomputed_quantities_fv",
@@ -148,6 +149,7 @@ def load_library(path: str = LIB_PATH):
     dpp = C.POINTER(C.c_double)
     lib.ca_ensemble_set_params.argtypes = [vp, dpp, dpp]
     lib.ca_ensemble_adjoint.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]
+    lib.ca_ensemble_adjoint_noalloc.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, vp, vp]
     lib.ca_dss_vjp.argtypes = [vp, vp, vp]
     lib.ca_remaining_tendency_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.ca_step_adjoint.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, vp]

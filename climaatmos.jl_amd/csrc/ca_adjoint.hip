@@ -958,17 +958,17 @@ __device__ void reverse_step(AdjSharedT<NF> &A, const DevGeom &g, const DevNum &
 // Adjoint of nsteps ensemble steps: workgroup b reverses member order[b].
 // Y0: initial states; YbN: cotangents of the states after nsteps steps.
 // Outputs: Yb0 (cotangent of Y0), D0b and awb per member.  ckpt / seg:
-// per-member scratch of (ceil(nsteps / every) + 1) and `every` states at
-// ckoff[m] / segoff[m].
+// scratch of (ceil(nsteps / every) + 1) and `every` ensemble states; member
+// m's part starts at that multiple of its state offset off[m] (the offsets
+// are the prefix sums of the member state sizes, so the parts are disjoint).
 template <int LPM, int NF>
 __global__ void __launch_bounds__(LPM) k_ensemble_adjoint(const DevGeom *__restrict__ mg, DevNum n,
                                                           const int *__restrict__ order, const int64_t *__restrict__ off,
                                                           const float *__restrict__ Y0, const float *__restrict__ YbN,
                                                           float *__restrict__ Yb0, float *__restrict__ D0b,
                                                           float *__restrict__ awb, float *__restrict__ ckpt,
-                                                          const int64_t *__restrict__ ckoff, float *__restrict__ seg,
-                                                          const int64_t *__restrict__ segoff, EnsTab tab, float dt,
-                                                          int nsteps, int every) {
+                                                          float *__restrict__ seg, EnsTab tab, float dt, int nsteps,
+                                                          int every) {
     using namespace adj;
     static_assert(LPM <= 128, "the adjoint runs at most 128 lanes per member");
     __shared__ AdjSharedT<NF> A;
@@ -978,8 +978,9 @@ __global__ void __launch_bounds__(LPM) k_ensemble_adjoint(const DevGeom *__restr
     const int nl = g.nlev;
     const int tid = threadIdx.x;
     const int64_t Sm = 5 * (int64_t)nl + 1;
-    float *ck = ckpt + ckoff[m];
-    float *sg = seg + segoff[m];
+    const int nseg = (nsteps + every - 1) / every;
+    float *ck = ckpt + (int64_t)(nseg + 1) * off[m];
+    float *sg = seg + (int64_t)every * off[m];
     auto load = [&](const float *src) {  // member state -> S.u (u_3 filtered)
         for (int k = tid; k < nl; k += LPM)
             for (int p = 0; p < 4; ++p) S.u[p][k] = src[p * nl + k];
@@ -1005,7 +1006,6 @@ __global__ void __launch_bounds__(LPM) k_ensemble_adjoint(const DevGeom *__restr
     for (int f = tid; f <= nl; f += LPM) A.ub[4][f] = (f == 0 || f == nl) ? 0.0f : ybn[4 * nl + f];
     sync();
     double kb = 0.0, ab = 0.0;
-    const int nseg = (nsteps + every - 1) / every;
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int len = min(every, nsteps - sgi * every);
         load(ck + sgi * Sm);
@@ -1043,8 +1043,7 @@ __global__ void __launch_bounds__(LPM) k_ensemble_adjoint(const DevGeom *__restr
 #define CA_ADJ_INST(LPM, NF)                                                                                   \
     template __global__ void k_ensemble_adjoint<LPM, NF>(const DevGeom *, DevNum, const int *, const int64_t *,         \
                                                          const float *, const float *, float *, float *, float *,     \
-                                                         float *, const int64_t *, float *, const int64_t *, EnsTab,   \
-                                                         float, int, int);
+                                                         float *, float *, EnsTab, float, int, int);
 CA_ADJ_INST(64, 33)
 CA_ADJ_INST(64, 65)
 CA_ADJ_INST(128, 65)

@@ -1,7 +1,13 @@
 // ca_ensemble_api.hip -- host side of the column ensemble (ca_ensemble_*).
 // Included after ca_api.hip in the single translation unit: it shares the
 // level-table and ARS343 helpers.
+#include <mutex>
+
 #include "ca_kernels.h"
+
+// ca_ensemble_adjoint_noalloc's scratch: 16 Mi floats (64 MiB) per GPU
+#define CA_NOALLOC_SCRATCH_FLOATS (int64_t(1) << 24)
+#define CA_MAX_DEVICES 64
 
 struct ca_ensemble {
     int device = 0;
@@ -31,8 +37,9 @@ struct ca_ensemble {
     std::vector<float> hcols;
     // adjoint scratch: checkpoints and segment states
     float *ckpt = nullptr, *segb = nullptr;
-    int64_t *ckoff = nullptr, *segoff = nullptr;
     int ck_nsteps = -1, ck_every = -1;
+    // this device's copy of the file-scope scratch array (ca_ensemble_adjoint_noalloc)
+    float *noalloc_scratch = nullptr;
 };
 
 namespace {
@@ -217,6 +224,54 @@ ca_status ens_create_impl(ca_ensemble *e, const ca_params *par, const ca_config 
     return CA_OK;
 }
 
+// Scratch of ca_ensemble_adjoint_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes its
+// name to a kernel (on the host the name is a stand-in, not a device
+// address): ca_ensemble_create resolves this device's address once with
+// hipGetSymbolAddress and every use goes through that pointer.
+static __device__ __align__(256) float scratch[CA_NOALLOC_SCRATCH_FLOATS];
+
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[CA_MAX_DEVICES] = {};
+
+// this device's address of `scratch`, resolved once per device with the
+// device current
+ca_status resolve_noalloc_scratch(ca_ensemble *e) {
+    int dev = 0;
+    EHIPCHK(e, hipGetDevice(&dev));
+    if (dev < 0 || dev >= CA_MAX_DEVICES) return efail(e, CA_ERR_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        EHIPCHK(e, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return efail(e, CA_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    e->noalloc_scratch = g_scratch_ptr[dev];
+    return CA_OK;
+}
+
+// the reverse launch shared by ca_ensemble_adjoint and its no-allocation
+// variant: ck holds (ceil(nsteps / every) + 1) * S floats, sg every * S
+ca_status ens_adjoint_launch(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int nsteps, int every,
+                             float *Yb0, float *D0b, float *awb, float *ck, float *sg, hipStream_t st) {
+    const int *ord = e->order[e->order_kind];
+    const dim3 grid(e->M);
+    // lanes: 64 for members of at most 32 levels; otherwise 64 or 128 (the
+    // 256-lane forward schedule runs its adjoint with 128)
+#define CA_ADJ_GO(LPM, NF)                                                                                       \
+    k_ensemble_adjoint<LPM, NF><<<grid, LPM, 0, st>>>(e->geoms, e->n, ord, e->doff, Y0, YbN, Yb0, D0b, awb, ck, sg, \
+                                                      e->tab, (float)dt, nsteps, every)
+    if (e->nf == 33) CA_ADJ_GO(64, 33);
+    else if (e->nf == 65 && e->lanes == 64) CA_ADJ_GO(64, 65);
+    else if (e->nf == 65) CA_ADJ_GO(128, 65);
+    else if (e->lanes == 64) CA_ADJ_GO(64, 129);
+    else CA_ADJ_GO(128, 129);
+#undef CA_ADJ_GO
+    EHIPCHK(e, hipGetLastError());
+    return CA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -231,7 +286,8 @@ ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, i
         return CA_ERR_ARG;
     }
     ca_ensemble *e = new ca_ensemble();
-    const ca_status s = ens_create_impl(e, params, config, nmembers, members);
+    ca_status s = ens_create_impl(e, params, config, nmembers, members);
+    if (s == CA_OK) s = resolve_noalloc_scratch(e);
     if (s != CA_OK) {
         g_create_error = e->err.empty() ? "ca_ensemble_create: invalid argument" : e->err;
         ca_ensemble_destroy(e);
@@ -244,7 +300,7 @@ ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, i
 ca_status ca_ensemble_destroy(ca_ensemble *e) {
     if (!e) return CA_ERR_ARG;
     ens_drop_graph(e);
-    for (void *p : {(void *)e->ckpt, (void *)e->segb, (void *)e->ckoff, (void *)e->segoff})
+    for (void *p : {(void *)e->ckpt, (void *)e->segb})
         if (p) (void)hipFree(p);
     for (void *p : e->allocs) (void)hipFree(p);
     delete e;
@@ -378,52 +434,38 @@ ca_status ca_ensemble_set_params(ca_ensemble *e, const double *alpha_w, const do
 ca_status ca_ensemble_adjoint(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int32_t nsteps,
                               int32_t every, float *Yb0, float *D0b, float *awb, void *stream) {
     if (!e || !Y0 || !YbN || !Yb0 || !D0b || !awb || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;
     if (e->ck_nsteps != nsteps || e->ck_every != every) {
         for (float **p : {&e->ckpt, &e->segb})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
             }
-        for (int64_t **p : {&e->ckoff, &e->segoff})
-            if (*p) {
-                (void)hipFree(*p);
-                *p = nullptr;
-            }
-        const int nseg = (nsteps + every - 1) / every;
-        std::vector<int64_t> ck(e->M), sg(e->M);
-        int64_t c = 0, g = 0;
-        for (int m = 0; m < e->M; ++m) {
-            const int64_t Sm = 5 * (int64_t)e->nlev[m] + 1;
-            ck[m] = c;
-            sg[m] = g;
-            c += (nseg + 1) * Sm;
-            g += every * Sm;
-        }
-        EHIPCHK(e, hipMalloc(&e->ckpt, c * sizeof(float)));
-        EHIPCHK(e, hipMalloc(&e->segb, g * sizeof(float)));
-        EHIPCHK(e, hipMalloc(&e->ckoff, e->M * sizeof(int64_t)));
-        EHIPCHK(e, hipMalloc(&e->segoff, e->M * sizeof(int64_t)));
-        EHIPCHK(e, hipMemcpy(e->ckoff, ck.data(), e->M * sizeof(int64_t), hipMemcpyHostToDevice));
-        EHIPCHK(e, hipMemcpy(e->segoff, sg.data(), e->M * sizeof(int64_t), hipMemcpyHostToDevice));
+        e->ck_nsteps = e->ck_every = -1;
+        const int64_t nseg = (nsteps + every - 1) / every;
+        EHIPCHK(e, hipMalloc(&e->ckpt, (size_t)((nseg + 1) * e->S) * sizeof(float)));
+        EHIPCHK(e, hipMalloc(&e->segb, (size_t)(every * e->S) * sizeof(float)));
         e->ck_nsteps = nsteps;
         e->ck_every = every;
     }
-    const int *ord = e->order[e->order_kind];
-    const dim3 grid(e->M);
-    // lanes: 64 for members of at most 32 levels; otherwise 64 or 128 (the
-    // 256-lane forward schedule runs its adjoint with 128)
-#define CA_ADJ_GO(LPM, NF)                                                                                         \
-    k_ensemble_adjoint<LPM, NF><<<grid, LPM, 0, st>>>(e->geoms, e->n, ord, e->doff, Y0, YbN, Yb0, D0b, awb, e->ckpt, \
-                                                      e->ckoff, e->segb, e->segoff, e->tab, (float)dt, nsteps, every)
-    if (e->nf == 33) CA_ADJ_GO(64, 33);
-    else if (e->nf == 65 && e->lanes == 64) CA_ADJ_GO(64, 65);
-    else if (e->nf == 65) CA_ADJ_GO(128, 65);
-    else if (e->lanes == 64) CA_ADJ_GO(64, 129);
-    else CA_ADJ_GO(128, 129);
-#undef CA_ADJ_GO
-    EHIPCHK(e, hipGetLastError());
-    return CA_OK;
+    return ens_adjoint_launch(e, Y0, YbN, dt, nsteps, every, Yb0, D0b, awb, e->ckpt, e->segb, (hipStream_t)stream);
+}
+
+ca_status ca_ensemble_adjoint_noalloc(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int32_t nsteps,
+                                      int32_t every, float *Yb0, float *D0b, float *awb, void *stream) {
+    if (!e || !Y0 || !YbN || !Yb0 || !D0b || !awb || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    if (!e->noalloc_scratch)
+        return efail(e, CA_ERR_DEVICE, "ca_ensemble_adjoint_noalloc: the scratch array of this device is not resolved");
+    const int64_t nseg = (nsteps + every - 1) / every;
+    const int64_t need = (nseg + 1 + every) * e->S;
+    if (need > CA_NOALLOC_SCRATCH_FLOATS)
+        return efail(e, CA_ERR_ARG,
+                     "ca_ensemble_adjoint_noalloc: needs (ceil(nsteps / every) + 1 + every) * state size = " +
+                         std::to_string(need) + " floats of scratch, above the maximum of " +
+                         std::to_string((long long)CA_NOALLOC_SCRATCH_FLOATS) +
+                         " (use ca_ensemble_adjoint, or change every)");
+    float *ck = e->noalloc_scratch;
+    float *sg = ck + (nseg + 1) * e->S;
+    return ens_adjoint_launch(e, Y0, YbN, dt, nsteps, every, Yb0, D0b, awb, ck, sg, (hipStream_t)stream);
 }
 
 }  // extern "C"

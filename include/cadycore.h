@@ -306,6 +306,18 @@ ca_status ca_ensemble_set_params(ca_ensemble *e, const double *alpha_w, const do
  * when every member has at most 32 levels or CA_ENS_LANES is 64, else 128. */
 ca_status ca_ensemble_adjoint(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int32_t nsteps,
                               int32_t every, float *Yb0, float *D0b, float *awb, void *stream);
+/* ca_ensemble_adjoint without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The checkpoints and segment states live in a file-scope device array of
+ * 2^24 floats (64 MiB) per GPU, held from the library's first use on that GPU
+ * until the process ends; it needs (ceil(nsteps / every) + 1 + every) *
+ * ca_ensemble_state_size(e) floats, and a larger request returns CA_ERR_ARG
+ * before any launch.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (on different streams, or from different ensembles or
+ * threads) race.  Same kernels and results as ca_ensemble_adjoint. */
+ca_status ca_ensemble_adjoint_noalloc(ca_ensemble *e, const float *Y0, const float *YbN, double dt, int32_t nsteps,
+                                      int32_t every, float *Yb0, float *D0b, float *awb, void *stream);
 
 /* ------------------------------------------------------------------------
  * Reverse mode of the sphere step (sphere grids; independent columns use

@@ -102,6 +102,11 @@ class Numerics:
     dt: float = 100.0
     # surface_flux_tendency! with the default AtmosSurface (src/types.jl:2292-2302):
     # ExchangeCoefficients(Cd = Ch = 0.0044), gustiness 1 m/s (surface_conditions.jl:254)
+    # vertical diffusion in the implicit tendency (diff_mode Implicit, YAML
+    # implicit_diffusion) and the iterations of the approximate linear solve it
+    # brings in (approximate_linear_solve_iters, default 1)
+    implicit_diffusion: bool = False
+    solve_iters: int = 1
     sfc_flux: bool = False
     Cd: float = 0.0044
     Ch: float = 0.0044

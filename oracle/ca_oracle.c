@@ -54,6 +54,8 @@ struct ora_ctx {
     real *Aur0, *Aur1, *Aue0, *Aue1;              /* face f: centres f-1, f */
     real *Auh0a, *Auh0b, *Auh1a, *Auh1b;          /* face f: (c=f-1)(1,2), (c=f)(1,2) */
     real *Auu_lo, *Auu_d, *Auu_up;
+    /* implicit diffusion: (rho e_tot, rho e_tot) and (u_h, u_h), tridiagonal on centres */
+    real *Aee_lo, *Aee_d, *Aee_up, *Ahh_lo, *Ahh_d, *Ahh_up;
     /* scratch */
     real *t[12];
     real *FE[4], *FI[4], *Ut, *U, *Rv, *dY;
@@ -122,7 +124,8 @@ int ora_create(const ora_geom *g, const ora_num *n, ora_ctx **out) {
     for (int i = 0; i < NQ; ++i)
         for (int k = 0; k < NQ; ++k) c->D[i][k] = (real)g->gll_D[4 * i + k];
     real **cen[] = {&c->cu3, &c->K, &c->T, &c->p, &c->h, &c->L1, &c->L2, &c->L3, &c->Ls,
-                    &c->Aru0, &c->Aru1, &c->Aeu0, &c->Aeu1};
+                    &c->Aru0, &c->Aru1, &c->Aeu0, &c->Aeu1, &c->Aee_lo, &c->Aee_d, &c->Aee_up,
+                    &c->Ahh_lo, &c->Ahh_d, &c->Ahh_up};
     for (size_t q = 0; q < sizeof(cen) / sizeof(cen[0]); ++q) *cen[q] = zalloc(sizeof(real) * c->nC);
     real **fac[] = {&c->fu3, &c->Aur0, &c->Aur1, &c->Aue0, &c->Aue1, &c->Auh0a, &c->Auh0b,
                     &c->Auh1a, &c->Auh1b, &c->Auu_lo, &c->Auu_d, &c->Auu_up};
@@ -146,6 +149,7 @@ void ora_destroy(ora_ctx *c) {
                     c->cu3, c->K, c->T, c->p, c->h, c->fu3, c->L1, c->L2, c->L3, c->Ls,
                     c->Aru0, c->Aru1, c->Aeu0, c->Aeu1, c->Aur0, c->Aur1, c->Aue0, c->Aue1,
                     c->Auh0a, c->Auh0b, c->Auh1a, c->Auh1b, c->Auu_lo, c->Auu_d, c->Auu_up,
+                    c->Aee_lo, c->Aee_d, c->Aee_up, c->Ahh_lo, c->Ahh_d, c->Ahh_up,
                     c->Ut, c->U, c->Rv, c->dY};
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); ++q) free(ptrs[q]);
     for (int q = 0; q < 12; ++q) free(c->t[q]);
@@ -505,6 +509,8 @@ static void vertical_transport(const ora_ctx *c, const real *rho, const real *ch
     }
 }
 
+static void vertical_diffusion_tendency(ora_ctx *c, real *Yt, real *Y);
+
 /* ------------------------------------------------------ implicit tendency */
 /* implicit_tendency!, implicit_tendency.jl:36-98 and :185-298 (dry subset) */
 static void implicit_tendency(ora_ctx *c, real *Yt, real *Y) {
@@ -551,8 +557,53 @@ static void implicit_tendency(ora_ctx *c, real *Yt, real *Y) {
                 real beta = alpha_w_of(c, col) * rayleigh_zeta(c, c->zf[f]);
                 U3(Yt)[IDX(f, col)] += -beta * U3(Y)[IDX(f, col)];
             }
+    /* vertical diffusion with diff_mode Implicit (:73-82) */
+    if (c->n.vert_diff && c->n.implicit_diffusion) vertical_diffusion_tendency(c, Yt, Y);
 }
 void ora_implicit_tendency(ora_ctx *c, ora_real *Yt, ora_real *Y) { implicit_tendency(c, (real *)Yt, (real *)Y); }
+
+/* update_diffusion_jacobian!, manual_sparse_jacobian.jl:1031-1261, dry,
+ * DecayWithHeightDiffusion (K_u = K_h = D0, harmonic face mean as in the
+ * tendency), for one column:
+ *   D = advdiv_matrix . diag(I^f(rho) / I^f(1 / max(K, eps))) . grad_matrix   (:1067-1068)
+ *   (rho e_tot, rho e_tot) = dtg D diag(cp_d / (cv_d rho)) - I                   (:1106-1108, :1138-1139)
+ *   (rho e_tot, rho) = 0                                                          (:1136-1137)
+ *   (u_h, u_h) = dtg diag(1 / rho) D - I                                           (:1253-1258)
+ * advdiv_matrix and grad_matrix are the centre divergence of a C3 face field
+ * with zero boundary flux and the face gradient of a centre field, so row k
+ * of D couples centres k - 1, k, k + 1 through faces k and k + 1. */
+static void diffusion_blocks(ora_ctx *c, real *Y, real dtg, int col) {
+    int nl = c->nl;
+    real *rho = RHO(Y);
+    real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
+    real Kh = (real)c->n.D0_col[col];
+    real Km = Kh > eps ? Kh : eps;
+    real cv = (real)c->n.cv_d, cp = (real)c->n.cp_d;
+    for (int k = 0; k < nl; ++k) {
+        int64_t q = IDX(k, col);
+        real Jc = c->Jc[q];
+        real dlo = 0, dup = 0;
+        if (k >= 1) { /* face k */
+            int f = k;
+            real rK = ((rho[IDX(f - 1, col)] + rho[q]) / 2) / ((1 / Km + 1 / Km) / 2);
+            real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
+            dlo = c->Jf[IDX(f, col)] * g33 * rK / Jc;
+        }
+        if (k + 1 <= nl - 1) { /* face k + 1 */
+            int f = k + 1;
+            real rK = ((rho[q] + rho[IDX(f, col)]) / 2) / ((1 / Km + 1 / Km) / 2);
+            real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
+            dup = c->Jf[IDX(f, col)] * g33 * rK / Jc;
+        }
+        real dd = -(dlo + dup);
+        c->Aee_lo[q] = k >= 1 ? dtg * dlo * (cp / (cv * rho[IDX(k - 1, col)])) : 0;
+        c->Aee_d[q] = dtg * dd * (cp / (cv * rho[q])) - 1;
+        c->Aee_up[q] = k + 1 <= nl - 1 ? dtg * dup * (cp / (cv * rho[IDX(k + 1, col)])) : 0;
+        c->Ahh_lo[q] = dtg * (1 / rho[q]) * dlo;
+        c->Ahh_d[q] = dtg * (1 / rho[q]) * dd - 1;
+        c->Ahh_up[q] = dtg * (1 / rho[q]) * dup;
+    }
+}
 
 /* --------------------------------------------------------------- Jacobian */
 /* update_advection_jacobian!, manual_sparse_jacobian.jl:713-870, dry, no
@@ -656,6 +707,7 @@ static void update_jacobian(ora_ctx *c, real *Y, real dtg) {
             c->Auu_d[q] = dtg * (pg_m * ka * dKa_0 + pg_p * kb * dKb_0 + (-beta)) - 1;
             c->Auu_up[q] = dtg * (pg_p * kb * dKb_p);
         }
+        if (c->n.vert_diff && c->n.implicit_diffusion) diffusion_blocks(c, Y, dtg, col);
     }
 }
 void ora_update_jacobian(ora_ctx *c, ora_real *Y, double dtgamma) { update_jacobian(c, (real *)Y, (real)dtgamma); }
@@ -713,7 +765,126 @@ static void jac_ldiv(ora_ctx *c, real *dY, const real *R) {
         }
     }
 }
-void ora_ldiv(ora_ctx *c, ora_real *dY, const ora_real *R) { jac_ldiv(c, (real *)dY, (const real *)R); }
+
+/* Thomas algorithm for one tridiagonal system of n rows (lo[0], up[n-1] unused) */
+static void tridiag(int n, const real *lo, const real *d, const real *up, const real *r, real *x, real *w) {
+    real m = d[0];
+    x[0] = r[0] / m;
+    for (int i = 1; i < n; ++i) {
+        w[i] = up[i - 1] / m;
+        m = d[i] - lo[i] * w[i];
+        x[i] = (r[i] - lo[i] * x[i - 1]) / m;
+    }
+    for (int i = n - 2; i >= 0; --i) x[i] -= w[i + 1] * x[i + 1];
+}
+
+/* ldiv! with implicit diffusion: ApproximateBlockArrowheadIterativeSolve(rho,
+ * rho e_tot; alg1 = BlockLowerTriangularSolve(rho), alg2 =
+ * BlockLowerTriangularSolve(u_h), P_alg1 = MainDiagonalPreconditioner(),
+ * n_iters) (manual_sparse_jacobian.jl:538-578), restated from ClimaCore
+ * 0.15.1's MatrixFields (not vendored): a Schur complement reduction on the
+ * scalars (rho, rho e_tot) whose scalar block A11 is inverted exactly, with
+ * the velocity Schur complement S = A22 - A21 inv(A11) A12 solved by
+ * n_iters stationary iterations x <- x - inv(P) (S x - b2') from x = 0,
+ * preconditioned by P = A22 - A21 inv(diag(A11)) A12 (tridiagonal on u_3),
+ * which is solved block lower triangularly (u_h, then u_3).  Without
+ * diffusion diag(A11) = A11 = -I and one iteration is the exact
+ * BlockArrowheadSolve of jac_ldiv. */
+static void jac_ldiv_approx(ora_ctx *c, real *dY, const real *R) {
+    const int nl = c->nl, nc = c->ncol, nf = nl + 1;
+    const int64_t nC = c->nC;
+    const int iters = c->n.solve_iters > 0 ? c->n.solve_iters : 1;
+    ORA_PAR_FOR
+    for (int col = 0; col < nc; ++col) {
+        real elo[nl], ed[nl], eup[nl], hlo[nl], hd[nl], hup[nl];
+        real br[nl], bh1[nl], bh2[nl], be[nl], bw[nf];
+        real ye[nl], xh1[nl], xh2[nl], xw[nf], rh1[nl], rh2[nl], rw[nf], zh1[nl], zh2[nl], zw[nf];
+        real slo[nf], sd[nf], sup[nf], g[nl], m[nl], wk[nf > nl ? nf : nl];
+#define CB(a, k) (c->a[IDX(k, col)])
+        for (int k = 0; k < nl; ++k) {
+            elo[k] = CB(Aee_lo, k); ed[k] = CB(Aee_d, k); eup[k] = CB(Aee_up, k);
+            hlo[k] = CB(Ahh_lo, k); hd[k] = CB(Ahh_d, k); hup[k] = CB(Ahh_up, k);
+            br[k] = R[IDX(k, col)];
+            bh1[k] = R[nC + IDX(k, col)];
+            bh2[k] = R[2 * nC + IDX(k, col)];
+            be[k] = R[3 * nC + IDX(k, col)];
+        }
+        for (int f = 0; f <= nl; ++f) bw[f] = R[4 * nC + IDX(f, col)];
+        /* y1 = inv(A11) b1: y_rho = -b_rho, A_ee y_e = b_e */
+        tridiag(nl, elo, ed, eup, be, ye, wk);
+        /* b2' = b2 - A21 y1 (u_3 rows only) */
+        for (int f = 1; f < nl; ++f)
+            bw[f] -= CB(Aur0, f) * (-br[f - 1]) + CB(Aur1, f) * (-br[f]) + CB(Aue0, f) * ye[f - 1] + CB(Aue1, f) * ye[f];
+        /* the preconditioner's u_3 block P_ww = A_ww - A_wrho (-1) A_rhow - A_we diag(A_ee)^-1 A_ew */
+        for (int f = 0; f <= nl; ++f) {
+            slo[f] = CB(Auu_lo, f);
+            sd[f] = CB(Auu_d, f);
+            sup[f] = CB(Auu_up, f);
+            if (f >= 1 && f <= nl - 1) {
+                int a = f - 1, b = f;
+                slo[f] += CB(Aur0, f) * CB(Aru0, a) - CB(Aue0, f) * CB(Aeu0, a) / ed[a];
+                sd[f] += CB(Aur0, f) * CB(Aru1, a) + CB(Aur1, f) * CB(Aru0, b) -
+                         (CB(Aue0, f) * CB(Aeu1, a) / ed[a] + CB(Aue1, f) * CB(Aeu0, b) / ed[b]);
+                sup[f] += CB(Aur1, f) * CB(Aru1, b) - CB(Aue1, f) * CB(Aeu1, b) / ed[b];
+            }
+        }
+        for (int k = 0; k < nl; ++k) xh1[k] = xh2[k] = 0;
+        for (int f = 0; f <= nl; ++f) xw[f] = 0;
+        for (int it = 0; it < iters; ++it) {
+            /* r = b2' - S x, S x = A22 x + A_wrho A_rhow x_w - A_we inv(A_ee) A_ew x_w */
+            for (int k = 0; k < nl; ++k) {
+                real s1 = hd[k] * xh1[k], s2 = hd[k] * xh2[k];
+                if (k >= 1) { s1 += hlo[k] * xh1[k - 1]; s2 += hlo[k] * xh2[k - 1]; }
+                if (k + 1 < nl) { s1 += hup[k] * xh1[k + 1]; s2 += hup[k] * xh2[k + 1]; }
+                rh1[k] = bh1[k] - s1;
+                rh2[k] = bh2[k] - s2;
+                g[k] = CB(Aeu0, k) * xw[k] + CB(Aeu1, k) * xw[k + 1];
+            }
+            tridiag(nl, elo, ed, eup, g, m, wk);
+            for (int f = 0; f <= nl; ++f) {
+                real sx = CB(Auu_d, f) * xw[f];
+                if (f >= 1) sx += CB(Auu_lo, f) * xw[f - 1];
+                if (f + 1 <= nl) sx += CB(Auu_up, f) * xw[f + 1];
+                if (f >= 1 && f <= nl - 1) {
+                    int a = f - 1, b = f;
+                    sx += CB(Auh0a, f) * xh1[a] + CB(Auh0b, f) * xh2[a] + CB(Auh1a, f) * xh1[b] + CB(Auh1b, f) * xh2[b];
+                    real pa = CB(Aru0, a) * xw[a] + CB(Aru1, a) * xw[a + 1];
+                    real pb = CB(Aru0, b) * xw[b] + CB(Aru1, b) * xw[b + 1];
+                    sx += CB(Aur0, f) * pa + CB(Aur1, f) * pb;
+                    sx -= CB(Aue0, f) * m[a] + CB(Aue1, f) * m[b];
+                }
+                rw[f] = bw[f] - sx;
+            }
+            /* z = inv(P) r: u_h first, then u_3 with the (u_3, u_h) coupling */
+            tridiag(nl, hlo, hd, hup, rh1, zh1, wk);
+            tridiag(nl, hlo, hd, hup, rh2, zh2, wk);
+            for (int f = 1; f < nl; ++f)
+                rw[f] -= CB(Auh0a, f) * zh1[f - 1] + CB(Auh0b, f) * zh2[f - 1] + CB(Auh1a, f) * zh1[f] +
+                         CB(Auh1b, f) * zh2[f];
+            tridiag(nf, slo, sd, sup, rw, zw, wk);
+            for (int k = 0; k < nl; ++k) { xh1[k] += zh1[k]; xh2[k] += zh2[k]; }
+            for (int f = 0; f <= nl; ++f) xw[f] += zw[f];
+        }
+        /* x1 = inv(A11) (b1 - A12 x2) */
+        for (int k = 0; k < nl; ++k) g[k] = be[k] - (CB(Aeu0, k) * xw[k] + CB(Aeu1, k) * xw[k + 1]);
+        tridiag(nl, elo, ed, eup, g, m, wk);
+        for (int k = 0; k < nl; ++k) {
+            int64_t q = IDX(k, col);
+            dY[q] = CB(Aru0, k) * xw[k] + CB(Aru1, k) * xw[k + 1] - br[k];
+            dY[nC + q] = xh1[k];
+            dY[2 * nC + q] = xh2[k];
+            dY[3 * nC + q] = m[k];
+        }
+        for (int f = 0; f <= nl; ++f) dY[4 * nC + IDX(f, col)] = xw[f];
+#undef CB
+    }
+}
+void ora_ldiv(ora_ctx *c, ora_real *dY, const ora_real *R) {
+    if (c->n.vert_diff && c->n.implicit_diffusion)
+        jac_ldiv_approx(c, (real *)dY, (const real *)R);
+    else
+        jac_ldiv(c, (real *)dY, (const real *)R);
+}
 
 void ora_column_blocks(ora_ctx *c, int col, double *Aru, double *Aeu, double *Aur, double *Aue,
                        double *Auu_lo, double *Auu_d, double *Auu_up) {
@@ -731,6 +902,26 @@ void ora_column_blocks(ora_ctx *c, int col, double *Aru, double *Aeu, double *Au
         Auu_lo[f] = c->Auu_lo[IDX(f, col)];
         Auu_d[f] = c->Auu_d[IDX(f, col)];
         Auu_up[f] = c->Auu_up[IDX(f, col)];
+    }
+}
+
+void ora_column_diffusion_blocks(ora_ctx *c, int col, double *Aee, double *Ahh) {
+    for (int k = 0; k < c->nl; ++k) {
+        Aee[3 * k] = c->Aee_lo[IDX(k, col)];
+        Aee[3 * k + 1] = c->Aee_d[IDX(k, col)];
+        Aee[3 * k + 2] = c->Aee_up[IDX(k, col)];
+        Ahh[3 * k] = c->Ahh_lo[IDX(k, col)];
+        Ahh[3 * k + 1] = c->Ahh_d[IDX(k, col)];
+        Ahh[3 * k + 2] = c->Ahh_up[IDX(k, col)];
+    }
+}
+
+void ora_column_uh_blocks(ora_ctx *c, int col, double *Auh) {
+    for (int f = 0; f <= c->nl; ++f) {
+        Auh[4 * f] = c->Auh0a[IDX(f, col)];
+        Auh[4 * f + 1] = c->Auh0b[IDX(f, col)];
+        Auh[4 * f + 2] = c->Auh1a[IDX(f, col)];
+        Auh[4 * f + 3] = c->Auh1b[IDX(f, col)];
     }
 }
 
@@ -854,6 +1045,81 @@ static void surface_flux_tendency(ora_ctx *c, real *Yt, real *Y) {
         RHOE(Yt)[q] -= div_e;
         UH1(Yt)[q] -= div_1 / rho_int;
         UH2(Yt)[q] -= div_2 / rho_int;
+    }
+}
+
+/* ------------------------------------------------- vertical diffusion */
+/* vertical_diffusion_boundary_layer_tendency!, vertical_diffusion_boundary_layer.jl:63-103,
+ * DecayWithHeightDiffusion(D0 = K_col, H = Inf): K_h = D0.  Added to Yt: by
+ * the explicit tendency (additional_tendency!, remaining_tendency.jl:185-190)
+ * with diff_mode Explicit, by the implicit tendency (implicit_tendency.jl:
+ * 73-82) with diff_mode Implicit.  Reads the precomputed T. */
+static void vertical_diffusion_tendency(ora_ctx *c, real *Yt, real *Y) {
+    int nl = c->nl, nc = c->ncol;
+    int sphere = c->g.nelem > 0;
+    real *rho = RHO(Y), *u1 = UH1(Y), *u2 = UH2(Y);
+    /* vertical_diffusion_boundary_layer_tendency!, vertical_diffusion_boundary_layer.jl:63-103,
+     * DecayWithHeightDiffusion(D0 = K_col, H = Inf): K_h = D0. */
+    real cpd = (real)c->n.cp_d, T0 = (real)c->n.T_0;
+    real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
+    const double *B = c->g.basis;
+    ORA_PAR_FOR
+    for (int col = 0; col < nc; ++col) {
+        real Kh = (real)c->n.D0_col[col];
+        real Km = Kh > eps ? Kh : eps;
+        /* s_d flux: -rhoK grad_v(s_d); diffdiv with zero boundary flux */
+        real *F = c->t[10], *Fu1 = c->t[0], *Fu2 = c->t[1], *Fu3 = c->t[2];
+        for (int f = 0; f <= nl; ++f) {
+            int64_t q = IDX(f, col);
+            F[q] = 0;
+            Fu1[q] = Fu2[q] = Fu3[q] = 0;
+            if (f == 0 || f == nl) continue;
+            int64_t a = IDX(f - 1, col), b = IDX(f, col);
+            real rK = ((rho[a] + rho[b]) / 2) / ((1 / Km + 1 / Km) / 2);
+            real sda = cpd * (c->T[a] - T0) + c->Phic[f - 1];
+            real sdb = cpd * (c->T[b] - T0) + c->Phic[f];
+            real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
+            F[q] = c->Jf[q] * (g33 * (-(rK * (sdb - sda))));
+            /* momentum: C3 (x) UVW strain; flux row of -2 rhoK S */
+            real ua[3], ub[3];
+            if (sphere) {
+                for (int d = 0; d < 3; ++d) {
+                    ua[d] = (u1[a] * (real)B[d * nc + col] + u2[a] * (real)B[(3 + d) * nc + col]) / c->sc[f - 1];
+                    ub[d] = (u1[b] * (real)B[d * nc + col] + u2[b] * (real)B[(3 + d) * nc + col]) / c->sc[f];
+                }
+            } else {
+                ua[0] = u1[a]; ua[1] = u2[a]; ua[2] = 0;
+                ub[0] = u1[b]; ub[1] = u2[b]; ub[2] = 0;
+            }
+            real inv = 1 / c->Jvf[f];
+            real fac = c->Jf[q] * inv * (-2 * rK) / 2;
+            Fu1[q] = fac * (ub[0] - ua[0]) * inv;
+            Fu2[q] = fac * (ub[1] - ua[1]) * inv;
+            Fu3[q] = fac * (ub[2] - ua[2]) * inv;
+        }
+        for (int k = 0; k < nl; ++k) {
+            int64_t q = IDX(k, col);
+            real Jc = c->Jc[q];
+            RHOE(Yt)[q] -= (F[IDX(k + 1, col)] - F[IDX(k, col)]) / Jc;
+            real d[3] = {(Fu1[IDX(k + 1, col)] - Fu1[IDX(k, col)]) / Jc,
+                         (Fu2[IDX(k + 1, col)] - Fu2[IDX(k, col)]) / Jc,
+                         (Fu3[IDX(k + 1, col)] - Fu3[IDX(k, col)]) / Jc};
+            real c1, c2;
+            if (sphere) {
+                c1 = c2 = 0;
+                for (int e = 0; e < 3; ++e) {
+                    c1 += d[e] * (real)B[(6 + e) * nc + col];
+                    c2 += d[e] * (real)B[(9 + e) * nc + col];
+                }
+                c1 *= c->sc[k];
+                c2 *= c->sc[k];
+            } else {
+                c1 = d[0];
+                c2 = d[1];
+            }
+            UH1(Yt)[q] -= c1 / rho[q];
+            UH2(Yt)[q] -= c2 / rho[q];
+        }
     }
 }
 
@@ -1090,71 +1356,7 @@ static void remaining_tendency(ora_ctx *c, real *Yt, real *Y) {
                 UH2(Yt)[IDX(k, col)] += -beta * u2[IDX(k, col)];
             }
         }
-    if (c->n.vert_diff) {
-        /* vertical_diffusion_boundary_layer_tendency!, vertical_diffusion_boundary_layer.jl:63-103,
-         * DecayWithHeightDiffusion(D0 = K_col, H = Inf): K_h = D0. */
-        real cpd = (real)c->n.cp_d, T0 = (real)c->n.T_0;
-        real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
-        const double *B = c->g.basis;
-        ORA_PAR_FOR
-        for (int col = 0; col < nc; ++col) {
-            real Kh = (real)c->n.D0_col[col];
-            real Km = Kh > eps ? Kh : eps;
-            /* s_d flux: -rhoK grad_v(s_d); diffdiv with zero boundary flux */
-            real *F = c->t[10], *Fu1 = c->t[0], *Fu2 = c->t[1], *Fu3 = c->t[2];
-            for (int f = 0; f <= nl; ++f) {
-                int64_t q = IDX(f, col);
-                F[q] = 0;
-                Fu1[q] = Fu2[q] = Fu3[q] = 0;
-                if (f == 0 || f == nl) continue;
-                int64_t a = IDX(f - 1, col), b = IDX(f, col);
-                real rK = ((rho[a] + rho[b]) / 2) / ((1 / Km + 1 / Km) / 2);
-                real sda = cpd * (c->T[a] - T0) + c->Phic[f - 1];
-                real sdb = cpd * (c->T[b] - T0) + c->Phic[f];
-                real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
-                F[q] = c->Jf[q] * (g33 * (-(rK * (sdb - sda))));
-                /* momentum: C3 (x) UVW strain; flux row of -2 rhoK S */
-                real ua[3], ub[3];
-                if (sphere) {
-                    for (int d = 0; d < 3; ++d) {
-                        ua[d] = (u1[a] * (real)B[d * nc + col] + u2[a] * (real)B[(3 + d) * nc + col]) / c->sc[f - 1];
-                        ub[d] = (u1[b] * (real)B[d * nc + col] + u2[b] * (real)B[(3 + d) * nc + col]) / c->sc[f];
-                    }
-                } else {
-                    ua[0] = u1[a]; ua[1] = u2[a]; ua[2] = 0;
-                    ub[0] = u1[b]; ub[1] = u2[b]; ub[2] = 0;
-                }
-                real inv = 1 / c->Jvf[f];
-                real fac = c->Jf[q] * inv * (-2 * rK) / 2;
-                Fu1[q] = fac * (ub[0] - ua[0]) * inv;
-                Fu2[q] = fac * (ub[1] - ua[1]) * inv;
-                Fu3[q] = fac * (ub[2] - ua[2]) * inv;
-            }
-            for (int k = 0; k < nl; ++k) {
-                int64_t q = IDX(k, col);
-                real Jc = c->Jc[q];
-                RHOE(Yt)[q] -= (F[IDX(k + 1, col)] - F[IDX(k, col)]) / Jc;
-                real d[3] = {(Fu1[IDX(k + 1, col)] - Fu1[IDX(k, col)]) / Jc,
-                             (Fu2[IDX(k + 1, col)] - Fu2[IDX(k, col)]) / Jc,
-                             (Fu3[IDX(k + 1, col)] - Fu3[IDX(k, col)]) / Jc};
-                real c1, c2;
-                if (sphere) {
-                    c1 = c2 = 0;
-                    for (int e = 0; e < 3; ++e) {
-                        c1 += d[e] * (real)B[(6 + e) * nc + col];
-                        c2 += d[e] * (real)B[(9 + e) * nc + col];
-                    }
-                    c1 *= c->sc[k];
-                    c2 *= c->sc[k];
-                } else {
-                    c1 = d[0];
-                    c2 = d[1];
-                }
-                UH1(Yt)[q] -= c1 / rho[q];
-                UH2(Yt)[q] -= c2 / rho[q];
-            }
-        }
-    }
+    if (c->n.vert_diff && !c->n.implicit_diffusion) vertical_diffusion_tendency(c, Yt, Y);
     if (c->n.sfc_flux) surface_flux_tendency(c, Yt, Y);
 }
 void ora_remaining_tendency(ora_ctx *c, ora_real *Yt, ora_real *Y) { remaining_tendency(c, (real *)Yt, (real *)Y); }
@@ -1247,7 +1449,10 @@ void ora_step(ora_ctx *c, ora_real *Yd, double dt, int32_t nsteps) {
             ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) c->Rv[q] = Ut[q] + dtg * c->Rv[q] - U[q];
             update_jacobian(c, U, dtg);
-            jac_ldiv(c, c->dY, c->Rv);
+            if (c->n.vert_diff && c->n.implicit_diffusion)
+                jac_ldiv_approx(c, c->dY, c->Rv);
+            else
+                jac_ldiv(c, c->dY, c->Rv);
             ORA_PAR_FOR
             for (int64_t q = 0; q < S; ++q) U[q] -= c->dY[q];
             /* post-Newton correction (T_post_imp!) after a cache_imp! refresh */

@@ -52,6 +52,8 @@ typedef struct ora_num {
     double Cd, Ch;         /* drag and heat exchange coefficients            */
     double gustiness;      /* wind-speed floor [m/s]                         */
     const double *T_sfc_col; /* per-column surface temperature [K]         */
+    int32_t implicit_diffusion; /* vertical diffusion in T_imp (diff_mode Implicit) */
+    int32_t solve_iters;   /* approximate_solve_iters of the linear solve    */
 } ora_num;
 
 #ifdef ORACLE_F32
@@ -101,6 +103,11 @@ void ora_op_wcurl12(ora_ctx *c, int k, const double *u3, double *o1, double *o2)
  * band blocks of update_jacobian, for one column (pinning tests) */
 void ora_column_blocks(ora_ctx *c, int col, double *Aru, double *Aeu, double *Aur,
                        double *Aue, double *Auu_lo, double *Auu_d, double *Auu_up);
+/* the implicit-diffusion blocks of one column: (rho e_tot, rho e_tot) and
+ * (u_h, u_h), tridiagonal on centres (lo, d, up each nlev) */
+void ora_column_diffusion_blocks(ora_ctx *c, int col, double *Aee, double *Ahh);
+/* the (u_3, u_h) blocks of one column: faces x (centre f-1: u_1, u_2; centre f: u_1, u_2) */
+void ora_column_uh_blocks(ora_ctx *c, int col, double *Auh);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ class _Num(C.Structure):
         ("D0_col", C.POINTER(C.c_double)), ("dt", C.c_double),
         ("sfc_flux", C.c_int32), ("Cd", C.c_double), ("Ch", C.c_double), ("gustiness", C.c_double),
         ("T_sfc_col", C.POINTER(C.c_double)),
+        ("implicit_diffusion", C.c_int32), ("solve_iters", C.c_int32),
     ]
 
 
@@ -98,6 +99,8 @@ class Oracle:
         for name in ("ora_op_div", "ora_op_wdiv", "ora_op_curl3", "ora_op_wcurl3"):
             getattr(lib, name).argtypes = [C.c_void_p, C.c_int, d, d, d]
         lib.ora_column_blocks.argtypes = [C.c_void_p, C.c_int] + [d] * 7
+        lib.ora_column_diffusion_blocks.argtypes = [C.c_void_p, C.c_int, d, d]
+        lib.ora_column_uh_blocks.argtypes = [C.c_void_p, C.c_int, d]
 
         from_geom = _import_geometry()
         xq, wq, D = from_geom.gll()
@@ -153,6 +156,8 @@ class Oracle:
                 raise ValueError("surface fluxes need T_sfc_col")
             num.Cd, num.Ch, num.gustiness = n.Cd, n.Ch, n.gustiness
             num.T_sfc_col = _dp(arr("Tsfc", T_sfc_col))
+        num.implicit_diffusion = int(bool(getattr(n, "implicit_diffusion", False)))
+        num.solve_iters = int(getattr(n, "solve_iters", 1))
         self._keep = keep
         self._geom = geom
         self._num = num
@@ -239,8 +244,13 @@ class Oracle:
         Aur, Aue = np.zeros(2 * (nl + 1)), np.zeros(2 * (nl + 1))
         lo, di, up = np.zeros(nl + 1), np.zeros(nl + 1), np.zeros(nl + 1)
         self.lib.ora_column_blocks(self.h, col, *[_dp(a) for a in (Aru, Aeu, Aur, Aue, lo, di, up)])
+        Aee, Ahh = np.zeros(3 * nl), np.zeros(3 * nl)
+        self.lib.ora_column_diffusion_blocks(self.h, col, _dp(Aee), _dp(Ahh))
+        Auh = np.zeros(4 * (nl + 1))
+        self.lib.ora_column_uh_blocks(self.h, col, _dp(Auh))
         return dict(Aru=Aru.reshape(nl, 2), Aeu=Aeu.reshape(nl, 2), Aur=Aur.reshape(nl + 1, 2),
-                    Aue=Aue.reshape(nl + 1, 2), lo=lo, di=di, up=up)
+                    Aue=Aue.reshape(nl + 1, 2), lo=lo, di=di, up=up, Aee=Aee.reshape(nl, 3),
+                    Ahh=Ahh.reshape(nl, 3), Auh=Auh.reshape(nl + 1, 4))
 
 
 def _import_geometry():

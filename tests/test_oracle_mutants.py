@@ -159,6 +159,30 @@ MUTANTS = [
      'const real f1 = g11 * fc1 + g12 * fc2, f2 = g12 * fc1 + g22 * fc2; /* C12 */',
      'const real f1 = fc1, f2 = fc2;',
      'sfc_sphere'),
+    ('implicit diffusion: cp_d / cv_d dropped from the (rho e, rho e) block',
+     'c->Aee_d[q] = dtg * dd * (cp / (cv * rho[q])) - 1;',
+     'c->Aee_d[q] = dtg * dd * (1 / rho[q]) - 1;',
+     'idiff_blocks'),
+    ('implicit diffusion: (u_h, u_h) block sign',
+     'c->Ahh_lo[q] = dtg * (1 / rho[q]) * dlo;',
+     'c->Ahh_lo[q] = -dtg * (1 / rho[q]) * dlo;',
+     'idiff_blocks'),
+    ('implicit diffusion: tendency left in the explicit tendency as well',
+     'if (c->n.vert_diff && !c->n.implicit_diffusion) vertical_diffusion_tendency(c, Yt, Y);',
+     'if (c->n.vert_diff) vertical_diffusion_tendency(c, Yt, Y);',
+     'idiff_move'),
+    ('approximate solve: Schur complement inv(A_ee) term sign',
+     'sx -= CB(Aue0, f) * m[a] + CB(Aue1, f) * m[b];',
+     'sx += CB(Aue0, f) * m[a] + CB(Aue1, f) * m[b];',
+     'idiff_solve'),
+    ('approximate solve: A21 inv(A11) b1 sign',
+     'bw[f] -= CB(Aur0, f) * (-br[f - 1]) + CB(Aur1, f) * (-br[f]) + CB(Aue0, f) * ye[f - 1] + CB(Aue1, f) * ye[f];',
+     'bw[f] += CB(Aur0, f) * (-br[f - 1]) + CB(Aur1, f) * (-br[f]) + CB(Aue0, f) * ye[f - 1] + CB(Aue1, f) * ye[f];',
+     'idiff_solve'),
+    ('approximate solve: back substitution without inv(A_ee)',
+     'dY[3 * nC + q] = m[k];',
+     'dY[3 * nC + q] = -g[k];',
+     'idiff_solve'),
     ('surface drag divided by the surface density',
      'UH1(Yt)[q] -= div_1 / rho_int;',
      'UH1(Yt)[q] -= div_1 / rho_sfc;',

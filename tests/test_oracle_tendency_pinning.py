@@ -24,6 +24,13 @@ restatement fails here.
   speed floored at the gustiness; on the deep sphere the column energy gain
   equals the bulk heat flux through the surface area and the covariant drag
   is parallel to the covariant wind.
+* Implicit vertical diffusion (diff_mode Implicit: implicit_tendency.jl:73-82,
+  the blocks of manual_sparse_jacobian.jl:1031-1261 and the approximate
+  arrowhead solve of :538-578): the tendency moves unchanged from the explicit
+  to the implicit tendency; the (rho e_tot, rho e_tot) and (u_h, u_h) blocks
+  equal central differences of the implicit tendency; the stationary
+  iterations of the linear solve converge to the dense solve of the assembled
+  column matrix, and without diffusion one iteration is that solve exactly.
 
 Each check is a function of an Oracle factory so that
 tests/test_oracle_mutants.py can run the same checks on copies of the oracle
@@ -379,6 +386,170 @@ def test_surface_flux_deep_sphere_budget():
     check_surface_flux_sphere(default_factory)
 
 
+# ------------------------------------------------ implicit vertical diffusion
+
+def _num_diff(implicit, iters=1, **kw):
+    return P.Numerics(dt=100.0, hyperdiff=False, vert_diff=True, implicit_diffusion=implicit, solve_iters=iters, **kw)
+
+
+def check_implicit_diffusion_moves_tendency(mk):
+    g = G.cubed_sphere(2, G.uniform_faces(8, 30000.0), PAR.planet_radius, PAR.Omega)
+    Y = M.Smooth(PAR).state(g)
+    D0 = np.random.default_rng(2).uniform(0.5, 50.0, g.ncol)
+
+    def both(implicit):
+        o = mk(g, PAR, _num_diff(implicit), D0_col=D0)
+        y = Y.copy()
+        o.set_precomputed(y)
+        return o.remaining_tendency(y), o.implicit_tendency(y)
+
+    te, ti = both(False)
+    ie, ii = both(True)
+    moved = te - ie
+    assert np.linalg.norm(moved) > 0
+    np.testing.assert_allclose(ii - ti, moved, rtol=0, atol=1e-12 * np.abs(moved).max())
+
+
+def _dense_column(b, nl):
+    """The assembled Jacobian of one column from its blocks, ordered
+    [rho | u_1 | u_2 | rho e_tot | u_3]."""
+    n = 5 * nl + 1
+    A = np.zeros((n, n))
+    r, h1, h2, e, w = (np.arange(nl), nl + np.arange(nl), 2 * nl + np.arange(nl), 3 * nl + np.arange(nl),
+                       4 * nl + np.arange(nl + 1))
+    A[r, r] = -1.0
+    for k in range(nl):
+        for blk, rows in (("Aee", e), ("Ahh", h1), ("Ahh", h2)):
+            lo, d, up = b[blk][k]
+            A[rows[k], rows[k]] = d
+            if k >= 1:
+                A[rows[k], rows[k - 1]] = lo
+            if k + 1 < nl:
+                A[rows[k], rows[k + 1]] = up
+        A[r[k], w[k]], A[r[k], w[k + 1]] = b["Aru"][k]
+        A[e[k], w[k]], A[e[k], w[k + 1]] = b["Aeu"][k]
+    for f in range(nl + 1):
+        A[w[f], w[f]] = b["di"][f]
+        if f >= 1:
+            A[w[f], w[f - 1]] = b["lo"][f]
+        if f + 1 <= nl:
+            A[w[f], w[f + 1]] = b["up"][f]
+        if 1 <= f <= nl - 1:
+            A[w[f], r[f - 1]], A[w[f], r[f]] = b["Aur"][f]
+            A[w[f], e[f - 1]], A[w[f], e[f]] = b["Aue"][f]
+            A[w[f], h1[f - 1]], A[w[f], h2[f - 1]], A[w[f], h1[f]], A[w[f], h2[f]] = b["Auh"][f]
+    return A
+
+
+def _column_planes(v, nl, nc, col):
+    nC = nl * nc
+    return np.concatenate([v[i * nC:(i + 1) * nC].reshape(nl, nc)[:, col] for i in range(4)]
+                          + [v[4 * nC:].reshape(nl + 1, nc)[:, col]])
+
+
+def check_implicit_diffusion_blocks(mk):
+    # a column at rest in u_3 (the advection blocks then do not depend on rho e_tot)
+    N, H, D0 = 12, 3000.0, 7.0
+    g = G.columns(1, G.tanh_stretched_faces(N, H, 100.0))
+    zc = g.zc
+    rng = np.random.default_rng(4)
+    rho = 1.2 * np.exp(-zc / 8000.0) * (1 + 0.01 * rng.random(N))
+    T = 290.0 - 0.0065 * zc + rng.random(N)
+    Y = _column_state(zc, rho, T, 3.0 + rng.standard_normal(N))
+    dtg = 40.0
+    o = mk(g, PAR, _num_diff(True), D0_col=np.array([D0]))
+    y = Y.copy()
+    o.set_precomputed(y)
+    o.update_jacobian(y, dtg)
+    b = o.column_blocks(0)
+
+    def Timp(yy):
+        yy = yy.copy()
+        o.set_precomputed(yy)
+        return o.implicit_tendency(yy)
+
+    for plane, blk in ((3, "Aee"), (1, "Ahh")):
+        for j in range(N):
+            v = np.zeros_like(Y)
+            v[plane * N + j] = 1.0
+            eps = 1e-4 * max(abs(Y[plane * N + j]), 1.0)
+            cd = dtg * (Timp(Y + eps * v) - Timp(Y - eps * v))[plane * N:(plane + 1) * N] / (2 * eps)
+            cd[j] -= 1.0
+            exp = np.zeros(N)
+            lo, d, up = b[blk][j, 1], b[blk][j, 1], b[blk][j, 1]
+            for k in range(N):  # column j of the tridiagonal block
+                if k == j:
+                    exp[k] = b[blk][k, 1]
+                elif k == j - 1:
+                    exp[k] = b[blk][k, 2]
+                elif k == j + 1:
+                    exp[k] = b[blk][k, 0]
+            assert np.abs(cd - exp).max() <= 1e-7 * np.abs(exp).max(), (blk, j, cd, exp)
+
+
+def check_implicit_diffusion_solve(mk):
+    rng = np.random.default_rng(6)
+    for kind in ("column", "sphere"):
+        if kind == "column":
+            g = G.columns(3, G.tanh_stretched_faces(16, 30000.0, 200.0))
+            Y = np.concatenate([_column_planes_state(g, rng)])
+            col = 1
+        else:
+            g = G.cubed_sphere(2, G.uniform_faces(10, 30000.0), PAR.planet_radius, PAR.Omega)
+            Y = M.Smooth(PAR).state(g)
+            col = 37
+        nl, nc = g.nlev, g.ncol
+        D0 = rng.uniform(10.0, 50.0, nc)
+        dtg = 20.0
+        R = rng.standard_normal(len(Y)) * np.maximum(np.abs(Y), 1.0) * 1e-3
+        errs = []
+        for iters in (1, 2, 4, 40):
+            o = mk(g, PAR, _num_diff(True, iters), D0_col=D0)
+            y = Y.copy()
+            o.set_precomputed(y)
+            o.update_jacobian(y, dtg)
+            if iters == 1:
+                A = _dense_column(o.column_blocks(col), nl)
+                exact = np.linalg.solve(A, _column_planes(R, nl, nc, col))
+            got = _column_planes(o.ldiv(R), nl, nc, col)
+            errs.append(np.linalg.norm(got - exact) / np.linalg.norm(exact))
+        assert errs[-1] < 1e-10 and errs[0] > 1e-8 and errs[1] < errs[0] and errs[2] < errs[1], (kind, errs)
+        # without diffusion one iteration is the exact block arrowhead solve
+        o = mk(g, PAR, P.Numerics(dt=100.0, hyperdiff=False), D0_col=D0)
+        y = Y.copy()
+        o.set_precomputed(y)
+        o.update_jacobian(y, dtg)
+        b = o.column_blocks(col)
+        b["Aee"] = b["Ahh"] = np.tile([0.0, -1.0, 0.0], (nl, 1))  # the fallback -I diagonals
+        A = _dense_column(b, nl)
+        exact = np.linalg.solve(A, _column_planes(R, nl, nc, col))
+        got = _column_planes(o.ldiv(R), nl, nc, col)
+        assert np.linalg.norm(got - exact) <= 1e-11 * np.linalg.norm(exact), kind
+
+
+def _column_planes_state(g, rng):
+    from cadycore import initial as I
+
+    Y = I.column_state(g, PAR, seed=3)
+    nl, nc = g.nlev, g.ncol
+    # w ~ 0.5 m/s on the faces (u_3 is covariant: w times the face spacing)
+    jvf = np.concatenate([[g.z_face[1] - g.z_face[0]], np.diff(g.zc), [g.z_face[-1] - g.z_face[-2]]])
+    Y[4 * nl * nc:] = (rng.normal(0.0, 0.5, (nl + 1, nc)) * jvf[:, None]).reshape(-1)
+    return Y
+
+
+def test_implicit_diffusion_moves_the_tendency():
+    check_implicit_diffusion_moves_tendency(default_factory)
+
+
+def test_implicit_diffusion_blocks_match_central_differences():
+    check_implicit_diffusion_blocks(default_factory)
+
+
+def test_approximate_arrowhead_solve_converges_to_the_dense_solve():
+    check_implicit_diffusion_solve(default_factory)
+
+
 ALL_CHECKS = {
     "mms_horizontal": check_mms_horizontal,
     "mms_vertical": check_mms_vertical,
@@ -389,4 +560,7 @@ ALL_CHECKS = {
     "rayleigh_uh": check_rayleigh_uh,
     "sfc_column": check_surface_flux_column,
     "sfc_sphere": check_surface_flux_sphere,
+    "idiff_move": check_implicit_diffusion_moves_tendency,
+    "idiff_blocks": check_implicit_diffusion_blocks,
+    "idiff_solve": check_implicit_diffusion_solve,
 }

@@ -55,6 +55,7 @@ class CaConfig(C.Structure):
         ("D0_col", C.POINTER(C.c_double)), ("dt", C.c_double),
         ("sfc_flux", C.c_int32), ("Cd", C.c_double), ("Ch", C.c_double), ("gustiness", C.c_double),
         ("T_sfc_col", C.POINTER(C.c_double)),
+        ("implicit_diffusion", C.c_int32), ("solve_iters", C.c_int32),
     ]
 
 
@@ -216,6 +217,8 @@ def create_args(grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None
             raise ValueError("surface fluxes need T_sfc_col")
         c.Cd, c.Ch, c.gustiness = numerics.Cd, numerics.Ch, numerics.gustiness
         c.T_sfc_col = _dp(arr("Tsfc", T_sfc_col))
+    c.implicit_diffusion = int(bool(numerics.implicit_diffusion))
+    c.solve_iters = int(numerics.solve_iters)
     return g, p, c, keep
 
 

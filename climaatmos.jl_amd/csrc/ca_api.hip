@@ -86,6 +86,12 @@ struct ca_handle {
     // native-layout staging of the FieldVector hooks (ca_*_fv) for arguments
     // in another layout: allocated by the first such call, kept until destroy
     float *fv_a = nullptr, *fv_b = nullptr;
+    // implicit vertical diffusion: the diffusion blocks and the solve's
+    // column scratch (9 centre + 7 face planes); F_I holds all five
+    // components then (fi_size = S), otherwise rho, rho e_tot and u_3
+    DiffBands E{};
+    float *idw = nullptr;
+    int64_t fi_size = 0;
 };
 
 // 32-bit pattern fill of n words (NaN poisoning and zeroing; a kernel, so that
@@ -158,7 +164,7 @@ ca_status poison_work(ca_handle *h, hipStream_t st) {
     for (int i = 0; i < 4; ++i)
         if ((s = fill_words(h, h->FE[i], h->S, kNaNBits, st))) return s;
     for (int i = 1; i < 4; ++i)
-        if ((s = fill_words(h, h->FI[i], 2 * g.nC + g.nF, kNaNBits, st))) return s;
+        if ((s = fill_words(h, h->FI[i], h->fi_size, kNaNBits, st))) return s;
     if ((s = fill_words(h, h->U, h->S, kNaNBits, st))) return s;
     return fill_words(h, h->raw, h->S, kNaNBits, st);
 }
@@ -314,7 +320,8 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
         const int ch = h->explicit_chunk;
         const int nchunks = ch > 0 ? (int)cdiv(g.nlev, ch) : 1;
         const dim3 grid(cdiv(g.ncol, 256), nchunks), blk(256);
-        const bool hy = n.hyper != 0, vd = n.vdiff != 0;
+        // with implicit diffusion the vertical diffusion is in the implicit tendency
+        const bool hy = n.hyper != 0, vd = n.vdiff != 0 && !n.idiff;
         if (n.sphere) {
             if (hy && !vd) k_explicit<true, true, false><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
             else if (!hy && !vd) k_explicit<true, false, false><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
@@ -624,19 +631,28 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     n.alpha_w = (float)cfg->alpha_w;
     n.vdiff = cfg->vert_diff;
     n.sflux = cfg->sfc_flux != 0;
+    n.idiff = cfg->implicit_diffusion && cfg->vert_diff;
+    n.siters = cfg->solve_iters > 0 ? cfg->solve_iters : 1;
     n.Cd = (float)cfg->Cd;
     n.Ch = (float)cfg->Ch;
     n.gust = (float)cfg->gustiness;
     n.dt = (float)cfg->dt;
     n.sphere = geom->nelem > 0;
     // work buffers
+    h->fi_size = n.idiff ? h->S : 2 * g.nC + g.nF;
+    if (n.idiff) {
+        float *eb = nullptr;
+        if ((s = dalloc(h, &eb, 6 * (size_t)g.nC))) return s;
+        h->E = DiffBands{eb, eb + g.nC, eb + 2 * g.nC, eb + 3 * g.nC, eb + 4 * g.nC, eb + 5 * g.nC};
+        if ((s = dalloc(h, &h->idw, 9 * (size_t)g.nC + 7 * (size_t)g.nF))) return s;
+    }
     if ((s = dalloc(h, &h->pre, 4 * (size_t)g.nC))) return s;
     if ((s = dalloc(h, &h->L, 4 * (size_t)g.nC))) return s;
     if ((s = dalloc(h, &h->scr, 2 * (size_t)g.nF + 6 * (size_t)g.nC))) return s;
     for (int i = 0; i < 4; ++i)
         if ((s = dalloc(h, &h->FE[i], (size_t)h->S))) return s;
     for (int i = 1; i < 4; ++i)
-        if ((s = dalloc(h, &h->FI[i], 2 * (size_t)g.nC + (size_t)g.nF))) return s;
+        if ((s = dalloc(h, &h->FI[i], (size_t)h->fi_size))) return s;
     if ((s = dalloc(h, &h->U, (size_t)h->S))) return s;
     if ((s = dalloc(h, &h->raw, (size_t)h->S))) return s;
     if ((s = dalloc(h, &h->flag, 1))) return s;
@@ -648,7 +664,7 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     HIPCHK(h, hipMemset(h->scr, 0xff, (2 * (size_t)g.nF + 6 * (size_t)g.nC) * sizeof(float)));
     for (int i = 0; i < 4; ++i) HIPCHK(h, hipMemset(h->FE[i], 0xff, (size_t)h->S * sizeof(float)));
     for (int i = 1; i < 4; ++i)
-        HIPCHK(h, hipMemset(h->FI[i], 0xff, (2 * (size_t)g.nC + (size_t)g.nF) * sizeof(float)));
+        HIPCHK(h, hipMemset(h->FI[i], 0xff, (size_t)h->fi_size * sizeof(float)));
     ars343(h);
     h->newton_variant = (nl + 1 <= 64) ? 1 : 0;
     // level chunks sized so that small grids still fill the chip: about
@@ -720,9 +736,41 @@ std::string check_create_args(const ca_geometry *geom, const ca_params *par, con
         return "upwinding must be 0 (none), 1 (first_order) or 3 (vanleer_limiter)";
     if (cfg->vert_diff && !cfg->D0_col) return "vertical diffusion needs D0_col";
     if (cfg->sfc_flux && !cfg->T_sfc_col) return "surface fluxes need T_sfc_col";
+    if (cfg->implicit_diffusion && !cfg->vert_diff) return "implicit diffusion needs vertical diffusion";
+    if (cfg->implicit_diffusion && (cfg->solve_iters < 1 || cfg->solve_iters > 100))
+        return "solve_iters must be in [1, 100]";
     if (cfg->sfc_flux && !(cfg->Cd >= 0 && cfg->Ch >= 0 && cfg->gustiness >= 0))
         return "surface fluxes need Cd, Ch and gustiness >= 0";
     return std::string();
+}
+
+// The implicit stage i with implicit vertical diffusion, composed from the
+// hook kernels as ClimaTimeSteppers composes it from the hooks: U~ into
+// h->raw, R = dtg T_imp(U~) (the Newton iteration starts at U~), the Jacobian
+// at U~, dU = solve(A, R) into U, U = U~ - dU, the post-Newton upwind
+// correction at U, and F_I = (U - U~) / dtg with all five components.
+ca_status idiff_stage(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const DevNum &n = h->n;
+    ca_status s;
+    if ((s = ensure_bands(h))) return s;
+    float *Ut = h->raw, *T = h->FI[i];
+    const unsigned nb = cdiv(h->S, 256);
+    const dim3 cl(cdiv(g.ncol, 128), g.nlev), fl(cdiv(g.ncol, 128), g.nlev + 1);
+    k_stage_sum<<<nb, 256, 0, st>>>(g, src, Ut);
+    k_implicit_tendency<<<fl, 128, 0, st>>>(g, n, T, Ut);
+    k_vdiff_add<<<cl, 128, 0, st>>>(g, n, Ut, T);
+    k_update_jacobian<<<fl, 128, 0, st>>>(g, n, Ut, dtg, h->B);
+    k_diffusion_jacobian<<<cl, 128, 0, st>>>(g, n, Ut, dtg, h->E);
+    k_ldiv_approx<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, h->E, n.siters, dtg, U, T, h->idw);
+    k_newton_update<<<nb, 256, 0, st>>>(g, Ut, U, U);
+    const float *C = nullptr;
+    if (n.upw != 0) {
+        k_correct<<<cl, 128, 0, st>>>(g, n, T, U);
+        C = T;
+    }
+    k_stage_finish<<<nb, 256, 0, st>>>(g, Ut, C, dtg, U, h->FI[i]);
+    return check_launch(h, "implicit-diffusion stage");
 }
 
 // Records of one step for the reverse pass: U_0 (the step's start state),
@@ -761,6 +809,14 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
                 ++src.nfi;
             }
         float dtg = (float)(dt * h->aI[i][i]);
+        if (h->n.idiff) {  // the implicit stage from the hook kernels (no records: no reverse mode)
+            {
+                Launch l(h, st, KID_NEWTON_STAGE);
+                if ((s = idiff_stage(h, src, dtg, i, h->U, st))) return s;
+            }
+            if ((s = explicit_tendency(h, h->FE[i], h->U, st, true, nullptr))) return s;
+            continue;
+        }
         const bool pcr = h->newton_variant == 1 || h->newton_variant == 2;
         if (rec && !pcr) {  // the PCR stage records U~ itself
             sadj::k_stage_state<<<dim3(cdiv(g.ncol, 256), g.nlev + 1), 256, 0, st>>>(g, src, rec_Ut(h, rec, i));
@@ -780,6 +836,7 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
     // u_{n+1} = u + dt sum b (F_E + F_I), then dss! and the filter
     FinalSrc fs{};
     fs.dt = (float)dt;
+    fs.fi_full = h->n.idiff;
     for (int j = 0; j < 4; ++j)
         if (h->b[j] != 0.0) {
             fs.fe[fs.nfe] = h->FE[j];
@@ -872,6 +929,7 @@ ca_status ca_implicit_tendency(ca_handle *h, float *Yt, const float *Y, void *st
     {
         Launch l(h, st, KID_HOOKS);
         k_implicit_tendency<<<dim3(cdiv(g.ncol, 128), g.nlev + 1), 128, 0, st>>>(g, h->n, Yt, Y);
+        if (h->n.idiff) k_vdiff_add<<<dim3(cdiv(g.ncol, 128), g.nlev), 128, 0, st>>>(g, h->n, Y, Yt);
     }
     return check_launch(h, "k_implicit_tendency");
 }
@@ -887,6 +945,8 @@ ca_status ca_update_jacobian(ca_handle *h, const float *Y, double dtgamma, void 
     {
         Launch l(h, st, KID_HOOKS);
         k_update_jacobian<<<dim3(cdiv(g.ncol, 128), g.nlev + 1), 128, 0, st>>>(g, h->n, Y, (float)dtgamma, h->B);
+        if (h->n.idiff)
+            k_diffusion_jacobian<<<dim3(cdiv(g.ncol, 128), g.nlev), 128, 0, st>>>(g, h->n, Y, (float)dtgamma, h->E);
     }
     return check_launch(h, "k_update_jacobian");
 }
@@ -900,7 +960,10 @@ ca_status ca_ldiv(ca_handle *h, float *dY, const float *R, void *stream) {
     if ((s = poison_work(h, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
-        k_ldiv<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, dY, R, h->scr);
+        if (h->n.idiff)
+            k_ldiv_approx<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, h->E, h->n.siters, 1.0f, dY, R, h->idw);
+        else
+            k_ldiv<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, dY, R, h->scr);
     }
     return check_launch(h, "k_ldiv");
 }

@@ -586,8 +586,8 @@ __global__ void __launch_bounds__(256) k_combine(DevGeom g, const float *u, floa
     const bool vec = (g.ncol & 3) == 0 && i0 + 3 < S;
     const int comp = i0 < 4 * g.nC ? (int)(i0 / g.nC) : 4;
     const int64_t r = i0 - (int64_t)comp * g.nC;
-    const bool has_fi = comp == 0 || comp == 3 || comp == 4;
-    const int ci = comp == 0 ? 0 : (comp == 3 ? 1 : 2);
+    const bool has_fi = s.fi_full || comp == 0 || comp == 3 || comp == 4;
+    const int ci = s.fi_full ? comp : (comp == 0 ? 0 : (comp == 3 ? 1 : 2));
     if (vec) {
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int t = 0; t < s.nfe; ++t) {
@@ -619,8 +619,8 @@ __global__ void __launch_bounds__(256) k_combine(DevGeom g, const float *u, floa
         const int64_t rr = idx - (int64_t)cp * g.nC;
         float acc = 0.0f;
         for (int t = 0; t < s.nfe; ++t) acc = fmaf(s.be[t], s.fe[t][idx], acc);
-        if (cp == 0 || cp == 3 || cp == 4) {
-            const int cj = cp == 0 ? 0 : (cp == 3 ? 1 : 2);
+        if (s.fi_full || cp == 0 || cp == 3 || cp == 4) {
+            const int cj = s.fi_full ? cp : (cp == 0 ? 0 : (cp == 3 ? 1 : 2));
             for (int t = 0; t < s.nfi; ++t) acc = fmaf(s.bi[t], s.fi[t][cj * g.nC + rr], acc);
         }
         out[idx] = u[idx] + s.dt * acc;

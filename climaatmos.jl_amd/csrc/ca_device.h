@@ -70,6 +70,7 @@ struct DevNum {
     float alpha_uh, alpha_w, dt;
     float Cd, Ch, gust;  // surface flux exchange coefficients and wind-speed floor
     int hyper, upw, rayleigh, vdiff, sphere, sflux;
+    int idiff, siters;  // vertical diffusion implicit; iterations of its linear solve
 };
 
 // ------------------------------------------------------------- launch order

@@ -285,6 +285,10 @@ ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, i
         g_create_error = "ca_ensemble_create: the column ensembles have no surface fluxes";
         return CA_ERR_ARG;
     }
+    if (config->implicit_diffusion) {
+        g_create_error = "ca_ensemble_create: the column ensembles have no implicit diffusion";
+        return CA_ERR_ARG;
+    }
     ca_ensemble *e = new ca_ensemble();
     ca_status s = ens_create_impl(e, params, config, nmembers, members);
     if (s == CA_OK) s = resolve_noalloc_scratch(e);

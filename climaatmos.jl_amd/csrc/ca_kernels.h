@@ -18,6 +18,12 @@ struct Bands {
     float *lo, *d, *up;
 };
 
+// implicit-diffusion blocks (tridiagonal on centres): (rho e_tot, rho e_tot)
+// and (u_h, u_h)
+struct DiffBands {
+    float *elo, *ed, *eup, *hlo, *hd, *hup;
+};
+
 #define CA_MAX_SRC 8
 // stage-state assembly: U~ = sum cf[t] * full[t] + sum ci[t] * fi[t]
 // (fi sources hold the rho, rho e_tot and u_3 planes of an implicit tendency)
@@ -36,6 +42,7 @@ struct FinalSrc {
     float bi[CA_MAX_SRC];
     int nfe, nfi;
     float dt;
+    int fi_full;  // 1: the fi sources hold all five components (implicit diffusion)
 };
 
 // column kernels (ca_column.hip)

@@ -277,6 +277,7 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
     if (!h || !Yb || !Ytb || !Y) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: the reverse mode has no surface fluxes");
+    if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: the reverse mode has no implicit diffusion");
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
     ca_status s;
@@ -305,6 +306,7 @@ ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int3
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_record: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode supports nlev <= 128");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no surface fluxes");
+    if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no implicit diffusion");
     hipStream_t st = (hipStream_t)stream;
     const int64_t S = h->S;
     const int nseg = (nsteps + every - 1) / every;
@@ -365,6 +367,7 @@ ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, doubl
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode supports nlev <= 128");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no surface fluxes");
+    if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no implicit diffusion");
     hipStream_t st = (hipStream_t)stream;
     ca_status s;
     if ((s = adj_ensure(h, every, (nsteps + every - 1) / every))) return s;

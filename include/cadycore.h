@@ -103,6 +103,14 @@ typedef struct ca_config {
     double Cd, Ch;          /* exchange coefficients (default 0.0044 each)   */
     double gustiness;       /* wind-speed floor [m/s] (default 1)            */
     const double *T_sfc_col; /* [ncol] surface temperature [K] when sfc_flux */
+    /* diff_mode Implicit (YAML implicit_diffusion: true): the vertical
+     * diffusion moves to the implicit tendency, the Jacobian gains its
+     * (rho e_tot, rho e_tot) and (u_h, u_h) blocks (update_diffusion_jacobian!,
+     * manual_sparse_jacobian.jl:1031-1261) and ldiv becomes the approximate
+     * block arrowhead solve with solve_iters iterations (:538-578).  Needs
+     * vert_diff; the reverse mode and the column ensembles return CA_ERR_ARG. */
+    int32_t implicit_diffusion;
+    int32_t solve_iters;    /* approximate_linear_solve_iters, 1 .. 100 (default 1) */
 } ca_config;
 
 /* Strided layout of one component: element (level v, node i, j, element h)
@@ -335,8 +343,8 @@ ca_status ca_dss_vjp(ca_handle *h, float *Yb, void *stream);
 ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, const float *Y, float *D0b,
                                     void *stream);
 /* The sphere reverse mode (ca_step_record, ca_step_pullback, ca_step_adjoint)
- * supports nlev <= 128 and no surface fluxes, and returns CA_ERR_ARG
- * otherwise, before any launch.
+ * supports nlev <= 128, no surface fluxes and explicit diffusion only, and
+ * returns CA_ERR_ARG otherwise, before any launch.
  * ca_step that also keeps what its pullback needs: runs nsteps steps of Y in
  * place (as ca_step) and stores the state at every `every`-th step in the
  * handle (ceil(nsteps / every) states; ca_step_pullback adds `every` steps of

@@ -53,6 +53,7 @@ struct CaConfig
     upwinding::Int32; rayleigh::Int32; zd_rayleigh::Float64; alpha_uh::Float64; alpha_w::Float64
     alpha_w_col::Ptr{Float64}; vert_diff::Int32; D0_col::Ptr{Float64}; dt::Float64
     sfc_flux::Int32; Cd::Float64; Ch::Float64; gustiness::Float64; T_sfc_col::Ptr{Float64}
+    implicit_diffusion::Int32; solve_iters::Int32
 end
 struct CaLayout
     offset::Int64; s_v::Int64; s_i::Int64; s_j::Int64; s_h::Int64

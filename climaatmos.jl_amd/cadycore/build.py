@@ -14,6 +14,9 @@ INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 LIB = os.path.join(_HERE, "libcadycore.so")
 HOST_LIB = os.path.join(_HERE, "libcadycore_host.so")
 HOST_SOURCES = ["ca_dss_table.cpp", "ca_host_tables.cpp"]
+IO_LIB = os.path.join(_HERE, "libcaio.so")
+IO_SOURCES = ["ca_io.cpp"]
+HDF5_ROOT = os.environ.get("CA_HDF5_ROOT", "/opt/conda")
 SOURCES = ["ca_unity.hip"]
 ARCH = os.environ.get("CA_OFFLOAD_ARCH", "gfx950")
 
@@ -40,6 +43,20 @@ def host_command(out=HOST_LIB):
     return ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", CSRC, "-o", out, *srcs]
 
 
+def io_command(out=IO_LIB):
+    """The checkpoint I/O library (host C++ over libhdf5)."""
+    srcs = [os.path.join(CSRC, s) for s in IO_SOURCES]
+    lib = os.path.join(HDF5_ROOT, "lib")
+    return ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INCLUDE,
+            "-I", os.path.join(HDF5_ROOT, "include"), "-o", out, *srcs,
+            f"-L{lib}", f"-Wl,-rpath,{lib}", "-lhdf5", "-static-libstdc++", "-static-libgcc"]
+
+
+def have_hdf5() -> bool:
+    return os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")) and any(
+        os.path.exists(os.path.join(HDF5_ROOT, "lib", n)) for n in ("libhdf5.so", "libhdf5.so.103"))
+
+
 def _run(cmd, out, verbose):
     tmp = out + ".tmp"
     cmd = [tmp if c == out else c for c in cmd]
@@ -54,6 +71,10 @@ def _run(cmd, out, verbose):
 def build(verbose: bool = False) -> str:
     _run(command(LIB), LIB, verbose)
     _run(host_command(HOST_LIB), HOST_LIB, verbose)
+    if have_hdf5():
+        _run(io_command(IO_LIB), IO_LIB, verbose)
+    else:
+        print(f"libcaio.so not built: no libhdf5 under {HDF5_ROOT} (set CA_HDF5_ROOT)", file=sys.stderr)
     return LIB
 
 

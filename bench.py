@@ -51,6 +51,8 @@ def parse():
                     help="run only the differentiation workload and print its figures (profiling runs)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the ragged column ensemble and print its figures (profiling runs)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the surface-flux and implicit-diffusion step timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
@@ -243,6 +245,43 @@ def copy_calibration(torch):
     torch.cuda.synchronize()
 
 
+def step_variants(torch, grid, par, Y0, dt, steps=30, warmup=5):
+    """ms per step of the same workload with the optional terms on: the surface
+    fluxes (ExchangeCoefficients Cd = Ch = 0.0044, the analytic surface
+    temperature; on in the reference's default AtmosModel) and vertical
+    diffusion (D0 = 5 m^2/s in every column) treated implicitly with one
+    iteration of the approximate arrowhead solve."""
+    import numpy as np
+
+    from cadycore import initial as I
+    from cadycore import params as P
+    from cadycore.lib import DryDycore
+
+    cases = {
+        "surface_flux": (dict(sfc_flux=True), dict(T_sfc_col=I.surface_temperature(grid))),
+        "implicit_vertical_diffusion": (dict(vert_diff=True, implicit_diffusion=True),
+                                        dict(D0_col=np.full(grid.ncol, 5.0))),
+        "surface_flux+implicit_vertical_diffusion": (
+            dict(sfc_flux=True, vert_diff=True, implicit_diffusion=True),
+            dict(T_sfc_col=I.surface_temperature(grid), D0_col=np.full(grid.ncol, 5.0))),
+    }
+    out = {}
+    for name, (nk, hk) in cases.items():
+        d = DryDycore(grid, par, P.Numerics(dt=dt, **nk), **hk)
+        Y = d.state(Y0)
+        d.step(Y, dt, warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.step(Y, dt, steps)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_step": round(el * 1e3, 4), "SYPD": round(dt / el / 365.0, 3), "steps": steps,
+                     "finite": bool(d.check_finite(Y))}
+        d.close()
+        del Y
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -319,6 +358,7 @@ def main():
 
     if a.copy_calibration:
         copy_calibration(torch)
+    variants = None if a.no_variants else step_variants(torch, grid, par, Y0, a.dt)
     ens = None if a.no_ensemble else ensemble_c3(torch, par)
     adj = None if a.no_adjoint else adjoint_c4(torch, par)
 
@@ -353,6 +393,7 @@ def main():
         "finite": bool(ok),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "step_variants": variants,
         "c3_ensemble": ens,
         "c4_adjoint": adj,
         "kernels": kernels,

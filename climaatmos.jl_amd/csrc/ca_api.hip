@@ -753,6 +753,19 @@ ca_status idiff_stage(ca_handle *h, const StageSrc &src, float dtg, int i, float
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
     ca_status s;
+    if (h->newton_variant >= 1) {  // the fused wavefront-per-column stage (nlev + 1 <= 64)
+        const unsigned nb = cdiv(g.ncol, 16);
+        float *FI = h->FI[i];
+        if (src.nfull == 2 && src.nfi == 0)
+            k_newton_idiff_pcr<2, 0><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
+        else if (src.nfull == 3 && src.nfi == 1)
+            k_newton_idiff_pcr<3, 1><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
+        else if (src.nfull == 4 && src.nfi == 2)
+            k_newton_idiff_pcr<4, 2><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
+        else
+            k_newton_idiff_pcr<-1, -1><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
+        return check_launch(h, "implicit-diffusion stage");
+    }
     if ((s = ensure_bands(h))) return s;
     float *Ut = h->raw, *T = h->FI[i];
     const unsigned nb = cdiv(h->S, 256);

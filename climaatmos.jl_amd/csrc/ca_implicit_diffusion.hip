@@ -256,6 +256,365 @@ __global__ void __launch_bounds__(128) k_ldiv_approx(DevGeom g, Bands A, DiffBan
     for (int f = 0; f <= nl; ++f) dY[4 * nC + (int64_t)f * s + col] = xw[(int64_t)f * s];
 }
 
+// ---------------------------------------------------------------------------
+// The whole implicit stage with implicit diffusion as one kernel, built like
+// k_newton_pcr: a workgroup assembles a tile of 16 columns into LDS, each
+// wavefront then takes one column with lane k = centre k and face k, and the
+// results leave through LDS.  Per column: U~, T_imp (advection, u_3 pressure
+// gradient and buoyancy, Rayleigh, and the vertical diffusion of rho e_tot and
+// u_h), the manual Jacobian and the diffusion blocks, the approximate
+// arrowhead solve, the Newton update, the upwind correction and
+// F_I = (U - U~) / dtg, all in registers.  The tridiagonal solves are parallel
+// cyclic reductions across the 64 lanes: float32 for the diffusion blocks
+// A_ee and A_hh (-I minus a diffusion operator: diagonally dominant), float64
+// for the u_3 Schur complement (stiff, as in the dry stage).  Same algebra as
+// the hook kernels above; the order of the roundings differs.
+
+namespace idp {
+
+// PCR on the 64 lane rows (row k couples to rows k -+ 1 through L and U;
+// L = 0 on row 0 and U = 0 on row 63): step s eliminates the couplings to
+// rows k -+ s with the neighbour rows divided by their own diagonal.
+__device__ __forceinline__ float solve_f32(float L, float D, float U, float r) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const float iD = 1.0f / D;
+        const float Ln = L * iD, Un = U * iD, rn = r * iD;
+        const float La = pcr::up(Ln, s), Ua = pcr::up(Un, s), ra = pcr::up(rn, s);
+        const float Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), rb = pcr::dn(rn, s);
+        const float nD = D - L * Ua - U * Lb;
+        const float nr = r - L * ra - U * rb;
+        L = -L * La;
+        U = -U * Ub;
+        D = nD;
+        r = nr;
+    }
+    return r / D;
+}
+// two right-hand sides of one matrix
+__device__ __forceinline__ void solve2_f32(float L, float D, float U, float &r1, float &r2) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const float iD = 1.0f / D;
+        const float Ln = L * iD, Un = U * iD, an = r1 * iD, bn = r2 * iD;
+        const float La = pcr::up(Ln, s), Ua = pcr::up(Un, s), aa = pcr::up(an, s), ba = pcr::up(bn, s);
+        const float Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), ab = pcr::dn(an, s), bb = pcr::dn(bn, s);
+        const float nD = D - L * Ua - U * Lb;
+        r1 = r1 - L * aa - U * ab;
+        r2 = r2 - L * ba - U * bb;
+        L = -L * La;
+        U = -U * Ub;
+        D = nD;
+    }
+    const float iD = 1.0f / D;
+    r1 *= iD;
+    r2 *= iD;
+}
+__device__ __forceinline__ float solve_f64(double Ld, double Dd, double Ud, double rd) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const double iD = ca_drcp(Dd);
+        const double Ln = Ld * iD, Un = Ud * iD, rn = rd * iD;
+        const double La = pcr::up(Ln, s), Ua = pcr::up(Un, s), ra = pcr::up(rn, s);
+        const double Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), rb = pcr::dn(rn, s);
+        const double nD = Dd - Ld * Ua - Ud * Lb;
+        const double nr = rd - Ld * ra - Ud * rb;
+        Ld = -Ld * La;
+        Ud = -Ud * Ub;
+        Dd = nD;
+        rd = nr;
+    }
+    return (float)(rd * ca_drcp(Dd));
+}
+
+}  // namespace idp
+
+template <int NF, int NI>
+__global__ void __launch_bounds__(256) k_newton_idiff_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U,
+                                                          float *FI) {
+    constexpr int TC = 16, R = TC + 1;
+    __shared__ float tU[5][64][R];
+    __shared__ float tF[5][64][R];
+    const int nl = g.nlev;
+    const int64_t nC = g.nC, nc = g.ncol;
+    const int col0 = ca_xcd_tile(blockIdx.x, gridDim.x) * TC;
+    const int tid = threadIdx.x;
+    const int nf = NF >= 0 ? NF : src.nfull;
+    const int ni = NI >= 0 ? NI : src.nfi;
+
+    // ---- phase 1: U~ = sum cf full + sum ci F_I over all five components
+    // (F_I carries u_h here), u_3 zero on the boundary faces
+    for (int idx = tid; idx < 64 * TC; idx += 256) {
+        const int r = idx / TC, cc = idx - r * TC;
+        const int col = col0 + cc;
+        const int64_t q = (int64_t)r * nc + col;
+        const bool ok = col < nc;
+#pragma unroll
+        for (int comp = 0; comp < 5; ++comp) {
+            const bool in = ok && (comp < 4 ? r < nl : (r >= 1 && r <= nl - 1));
+            float v = 0.0f;
+            if (in) {
+                const int64_t o = comp * nC + q;
+#pragma unroll
+                for (int t = 0; t < (NF >= 0 ? NF : CA_MAX_SRC); ++t)
+                    if (t < nf) v = fmaf(src.cf[t], src.full[t][o], v);
+#pragma unroll
+                for (int t = 0; t < (NI >= 0 ? NI : CA_MAX_SRC); ++t)
+                    if (t < ni) v = fmaf(src.ci[t], src.fi[t][o], v);
+            }
+            tU[comp][r][cc] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: one column per wavefront
+    const int k = tid & 63;
+    const int wid = tid >> 6;
+    const bool hasC = k < nl;
+    const bool intF = k >= 1 && k <= nl - 1;
+    const bool isF = k <= nl;
+    const int kc = hasC ? k : nl - 1;
+    const int kf = isF ? k : nl;
+    const float g33 = g.g33f[kf];
+    const float g33p = g.g33f[kc + 1];
+    const float kcv = n.cp_d / n.cv_d;
+    for (int cc = wid; cc < TC; cc += 4) {
+        const int col = col0 + cc;
+        if (col >= nc) break;  // uniform across the wavefront
+        const float rho = tU[0][kc][cc], u1 = tU[1][kc][cc], u2 = tU[2][kc][cc], rhoe = tU[3][kc][cc];
+        const float u3 = intF ? tU[4][k][cc] : 0.0f;
+        const float u3p = pcr::dn(u3, 1);
+        const float u3m = pcr::up(u3, 1);
+        const CV c = cv_make(g, n, kc, col, rho, u1, u2, rhoe, u3, hasC ? u3p : 0.0f);
+        const float rJ = c.rho * c.J;
+        const float rJm = pcr::up(rJ, 1), hm = pcr::up(c.h, 1);
+        const float rJp = pcr::dn(rJ, 1), hp = pcr::dn(c.h, 1);
+        // centre row k: A(rho,u3), A(rhoe,u3) and the advective residuals
+        float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f, Rr = 0.0f, Te = 0.0f;
+        if (hasC) {
+            float Fm = 0.0f, Fp = 0.0f, Hm = 0.0f, Hp = 0.0f;
+            if (k >= 1) {
+                const float rJf = 0.5f * (rJm + rJ);
+                a0 = dtg * (rJf * c.iJ) * g33;
+                e0 = a0 * (0.5f * (hm + c.h));
+                Fm = rJf * (u3 * g33);
+                Hm = Fm * (0.5f * (hm + c.h));
+            }
+            if (k + 1 <= nl - 1) {
+                const float rJf = 0.5f * (rJ + rJp);
+                a1 = dtg * (-rJf * c.iJ) * g33p;
+                e1 = a1 * (0.5f * (c.h + hp));
+                Fp = rJf * (u3p * g33p);
+                Hp = Fp * (0.5f * (c.h + hp));
+            }
+            Rr = dtg * (-(Fp - Fm) * c.iJ);
+            Te = -(Hp - Hm) * c.iJ;
+        }
+        // centre k - 1 (lane k - 1)
+        CV a;
+        a.rho = pcr::up(c.rho, 1);
+        a.u1 = pcr::up(c.u1, 1);
+        a.u2 = pcr::up(c.u2, 1);
+        a.K = pcr::up(c.K, 1);
+        a.Td = pcr::up(c.Td, 1);
+        a.pd = pcr::up(c.pd, 1);
+        a.Pid = pcr::up(c.Pid, 1);
+        a.T = (float)a.Td;
+        a.p = (float)a.pd;
+        a.Pi = (float)a.Pid;
+        a.ct1 = pcr::up(c.ct1, 1);
+        a.ct2 = pcr::up(c.ct2, 1);
+        // vertical diffusion: fluxes through face k, the tendency of centre k
+        float Fs = 0.0f, Fu0 = 0.0f, Fu1 = 0.0f, Fu2 = 0.0f;
+        if (intF) vdiff_face(g, n, col, k, a, c, Fs, Fu0, Fu1, Fu2);
+        const float Fs1 = pcr::dn(Fs, 1), Fu01 = pcr::dn(Fu0, 1), Fu11 = pcr::dn(Fu1, 1), Fu21 = pcr::dn(Fu2, 1);
+        float Rh1 = 0.0f, Rh2 = 0.0f;
+        if (hasC) {
+            const float iJc = c.iJ;
+            Te -= (Fs1 - Fs) * iJc;
+            const float d0 = (Fu01 - Fu0) * iJc, d1 = (Fu11 - Fu1) * iJc, d2 = (Fu21 - Fu2) * iJc;
+            float c1, c2;
+            if (n.sphere) {
+                const float *B = g.basis;
+                const float sk = g.sc[kc];
+                c1 = sk * (d0 * B[6 * nc + col] + d1 * B[7 * nc + col] + d2 * B[8 * nc + col]);
+                c2 = sk * (d0 * B[9 * nc + col] + d1 * B[10 * nc + col] + d2 * B[11 * nc + col]);
+            } else {
+                c1 = d0;
+                c2 = d1;
+            }
+            Rh1 = dtg * (0.0f - c1 / c.rho);
+            Rh2 = dtg * (0.0f - c2 / c.rho);
+        }
+        const float Re = dtg * Te;
+        // diffusion blocks of centre k (rows >= nl: identity)
+        float elo = 0.0f, ed = 1.0f, eup = 0.0f, hlo = 0.0f, hd = 1.0f, hup = 0.0f;
+        if (hasC) {
+            const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
+            const float Kf = 1.0f / (0.5f * (1.0f / Kh + 1.0f / Kh));
+            const float rm = a.rho, rp = pcr::dn(c.rho, 1), r0 = c.rho;
+            const float Jc = c.J;
+            float dlo = 0.0f, dup = 0.0f;
+            if (k >= 1) {
+                const float sf = g.sf[k];
+                dlo = (g.Jh[col] * sf * sf * g.Jvf[k]) * g.g33f[k] * ((0.5f * (rm + r0)) * Kf) / Jc;
+            }
+            if (k + 1 <= nl - 1) {
+                const float sf = g.sf[k + 1];
+                dup = (g.Jh[col] * sf * sf * g.Jvf[k + 1]) * g.g33f[k + 1] * ((0.5f * (r0 + rp)) * Kf) / Jc;
+            }
+            const float dd = -(dlo + dup);
+            elo = k >= 1 ? dtg * dlo * (kcv / rm) : 0.0f;
+            ed = dtg * dd * (kcv / r0) - 1.0f;
+            eup = k + 1 <= nl - 1 ? dtg * dup * (kcv / rp) : 0.0f;
+            hlo = dtg * (1.0f / r0) * dlo;
+            hd = dtg * (1.0f / r0) * dd - 1.0f;
+            hup = dtg * (1.0f / r0) * dup;
+        }
+        // face row k and its residual
+        FaceRow fr;
+        fr.r0 = fr.r1 = fr.e0 = fr.e1 = fr.h0a = fr.h0b = fr.h1a = fr.h1b = fr.lo = fr.up = 0.0f;
+        fr.d = 1.0f;
+        float Ru3 = 0.0f;
+        if (intF) {
+            fr = jac_face_row(g, n, k, col, a, c, u3m, u3, u3p, dtg);
+            Ru3 = dtg * (pgf_u3(g, n, k, a, c) - beta_w(g, n, k, col) * u3);
+        } else if (isF) {
+            fr.d = -dtg * beta_w(g, n, k, col) - 1.0f;
+        }
+        // ---- ApproximateBlockArrowheadIterativeSolve
+        // y_e = inv(A_ee) b_e; b_w' = b_w - A_wrho y_rho - A_we y_e (y_rho = -b_rho)
+        const float ye = idp::solve_f32(hasC ? elo : 0.0f, hasC ? ed : 1.0f, hasC ? eup : 0.0f, hasC ? Re : 0.0f);
+        const float pye = pcr::up(ye, 1), pRr = pcr::up(Rr, 1);
+        const float pa0 = pcr::up(a0, 1), pa1 = pcr::up(a1, 1), pe0 = pcr::up(e0, 1), pe1 = pcr::up(e1, 1);
+        const float ped = pcr::up(ed, 1);
+        float bw = Ru3, Pl = fr.lo, Pd = fr.d, Pu = fr.up;
+        if (intF) {
+            bw = Ru3 + fr.r0 * pRr + fr.r1 * Rr - (fr.e0 * pye + fr.e1 * ye);
+            Pl = fr.lo + fr.r0 * pa0 - fr.e0 * pe0 / ped;
+            Pd = fr.d + fr.r0 * pa1 + fr.r1 * a0 - (fr.e0 * pe1 / ped + fr.e1 * e0 / ed);
+            Pu = fr.up + fr.r1 * a1 - fr.e1 * e1 / ed;
+        }
+        if (!isF) {
+            Pl = 0.0f;
+            Pd = 1.0f;
+            Pu = 0.0f;
+            bw = 0.0f;
+        }
+        // stationary iterations from x = 0, preconditioned by the complement
+        // with diag(A_ee): u_h first, then u_3
+        float xh1 = 0.0f, xh2 = 0.0f, xw = 0.0f;
+        for (int it = 0; it < n.siters; ++it) {
+            float rh1 = hasC ? Rh1 : 0.0f, rh2 = hasC ? Rh2 : 0.0f, rw = bw;
+            if (it > 0) {  // r = b' - S x
+                const float x1m = pcr::up(xh1, 1), x1p = pcr::dn(xh1, 1);
+                const float x2m = pcr::up(xh2, 1), x2p = pcr::dn(xh2, 1);
+                const float xwm = pcr::up(xw, 1), xwp = pcr::dn(xw, 1);
+                if (hasC) {
+                    rh1 -= hd * xh1 + (k >= 1 ? hlo * x1m : 0.0f) + (k + 1 < nl ? hup * x1p : 0.0f);
+                    rh2 -= hd * xh2 + (k >= 1 ? hlo * x2m : 0.0f) + (k + 1 < nl ? hup * x2p : 0.0f);
+                }
+                const float gg = hasC ? e0 * xw + e1 * xwp : 0.0f;  // A_ew x_w
+                const float mm = idp::solve_f32(hasC ? elo : 0.0f, hasC ? ed : 1.0f, hasC ? eup : 0.0f, gg);
+                const float pmm = pcr::up(mm, 1);
+                const float pA = a0 * xw + a1 * xwp;  // A_rho,w x_w at centre k
+                const float ppA = pcr::up(pA, 1);
+                float sx = fr.d * xw + (k >= 1 ? fr.lo * xwm : 0.0f) + (k + 1 <= nl ? fr.up * xwp : 0.0f);
+                if (intF) {
+                    sx += fr.h0a * x1m + fr.h0b * x2m + fr.h1a * xh1 + fr.h1b * xh2;
+                    sx += fr.r0 * ppA + fr.r1 * pA;
+                    sx -= fr.e0 * pmm + fr.e1 * mm;
+                }
+                if (isF) rw = bw - sx;
+            }
+            // z_h = inv(A_hh) r_h (two components); z_w = inv(P_ww) (r_w - A_wh z_h)
+            idp::solve2_f32(hasC ? hlo : 0.0f, hasC ? hd : 1.0f, hasC ? hup : 0.0f, rh1, rh2);
+            const float z1m = pcr::up(rh1, 1), z2m = pcr::up(rh2, 1);
+            if (intF) rw -= fr.h0a * z1m + fr.h0b * z2m + fr.h1a * rh1 + fr.h1b * rh2;
+            const float zw = idp::solve_f64(Pl, Pd, Pu, rw);
+            if (hasC) {
+                xh1 += rh1;
+                xh2 += rh2;
+            }
+            if (isF) xw += zw;
+        }
+        // x_1 = inv(A_11) (b_1 - A_12 x_2)
+        const float xwp = pcr::dn(xw, 1);
+        const float gg = hasC ? Re - (e0 * xw + e1 * xwp) : 0.0f;
+        const float xe = idp::solve_f32(hasC ? elo : 0.0f, hasC ? ed : 1.0f, hasC ? eup : 0.0f, gg);
+        const float xr = a0 * xw + a1 * xwp - Rr;
+        // Newton update
+        const float rho_n = rho - xr;
+        const float u1_n = u1 - xh1, u2_n = u2 - xh2;
+        float rhoe_n = rhoe - xe;
+        const float u3_n = intF ? u3 - xw : 0.0f;
+        // post-Newton upwind correction at the new state
+        if (n.upw != 0) {
+            const float u3_np = pcr::dn(u3_n, 1);
+            const float hn = cv_enthalpy(g, n, kc, col, rho_n, u1_n, u2_n, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
+            const float href = __shfl(hn, 0, 64);
+            const float hr = hn - href;
+            const float sk = g.sc[kc];
+            const float rJn = rho_n * (g.Jh[col] * sk * sk * g.Jvc[kc]);
+            const float h2 = pcr::up(hr, 2), h1 = pcr::up(hr, 1), h4 = pcr::dn(hr, 1);
+            const float rJn1 = pcr::up(rJn, 1);
+            float dF = 0.0f;
+            if (intF) {
+                const float v = u3_n * g33;
+                const float rJf = 0.5f * (rJn1 + rJn);
+                const float upw = upwind_product(n.upw, k, nl, v, n.dt, h2, h1, hr, h4);
+                const float cen = upwind_product(0, k, nl, v, n.dt, 0.0f, h1, hr, 0.0f);
+                dF = rJf * upw - rJf * cen;
+            }
+            const float dF1 = pcr::dn(dF, 1);
+            const float corr = hasC ? -(dF1 - dF) * (g.iJh[col] * g.isJc[kc]) : 0.0f;
+            rhoe_n += dtg * corr;
+        }
+        // U and F_I = (U - U~) / dtg
+        if (hasC) {
+            tU[0][k][cc] = rho_n;
+            tU[1][k][cc] = u1_n;
+            tU[2][k][cc] = u2_n;
+            tU[3][k][cc] = rhoe_n;
+            tF[0][k][cc] = (rho_n - rho) / dtg;
+            tF[1][k][cc] = (u1_n - u1) / dtg;
+            tF[2][k][cc] = (u2_n - u2) / dtg;
+            tF[3][k][cc] = (rhoe_n - rhoe) / dtg;
+        }
+        if (isF) {
+            tU[4][k][cc] = u3_n;
+            tF[4][k][cc] = (u3_n - u3) / dtg;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: coalesced stores of U and F_I
+    for (int idx = tid; idx < 64 * TC; idx += 256) {
+        const int r = idx / TC, cc = idx - r * TC;
+        const int col = col0 + cc;
+        if (col >= nc) continue;
+        const int64_t q = (int64_t)r * nc + col;
+        if (r < nl) {
+#pragma unroll
+            for (int comp = 0; comp < 4; ++comp) {
+                U[comp * nC + q] = tU[comp][r][cc];
+                FI[comp * nC + q] = tF[comp][r][cc];
+            }
+        }
+        if (r <= nl) {
+            U[4 * nC + q] = tU[4][r][cc];
+            FI[4 * nC + q] = tF[4][r][cc];
+        }
+    }
+}
+
+#define CA_INST_IDIFF(NF, NI)                                                                             \
+    template __global__ void k_newton_idiff_pcr<NF, NI>(DevGeom, DevNum, StageSrc, float, float *, float *);
+CA_INST_IDIFF(2, 0)
+CA_INST_IDIFF(3, 1)
+CA_INST_IDIFF(4, 2)
+CA_INST_IDIFF(-1, -1)
+
 // U~ = sum cf full + sum ci fi over all five components (the implicit
 // tendencies carry u_h here), u_3 zero on the boundary faces
 __global__ void __launch_bounds__(256) k_stage_sum(DevGeom g, StageSrc src, float *out) {

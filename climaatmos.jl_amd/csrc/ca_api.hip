@@ -749,21 +749,33 @@ std::string check_create_args(const ca_geometry *geom, const ca_params *par, con
 // h->raw, R = dtg T_imp(U~) (the Newton iteration starts at U~), the Jacobian
 // at U~, dU = solve(A, R) into U, U = U~ - dU, the post-Newton upwind
 // correction at U, and F_I = (U - U~) / dtg with all five components.
+template <bool V4>
+void launch_idiff_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const unsigned nb = cdiv(g.ncol, 16);
+    float *FI = h->FI[i];
+    if (src.nfull == 2 && src.nfi == 0)
+        k_newton_idiff_pcr<2, 0, V4><<<nb, 256, 0, st>>>(g, h->n, src, dtg, U, FI);
+    else if (src.nfull == 3 && src.nfi == 1)
+        k_newton_idiff_pcr<3, 1, V4><<<nb, 256, 0, st>>>(g, h->n, src, dtg, U, FI);
+    else if (src.nfull == 4 && src.nfi == 2)
+        k_newton_idiff_pcr<4, 2, V4><<<nb, 256, 0, st>>>(g, h->n, src, dtg, U, FI);
+    else
+        k_newton_idiff_pcr<-1, -1, V4><<<nb, 256, 0, st>>>(g, h->n, src, dtg, U, FI);
+}
+
 ca_status idiff_stage(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
     ca_status s;
     if (h->newton_variant >= 1) {  // the fused wavefront-per-column stage (nlev + 1 <= 64)
-        const unsigned nb = cdiv(g.ncol, 16);
-        float *FI = h->FI[i];
-        if (src.nfull == 2 && src.nfi == 0)
-            k_newton_idiff_pcr<2, 0><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
-        else if (src.nfull == 3 && src.nfi == 1)
-            k_newton_idiff_pcr<3, 1><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
-        else if (src.nfull == 4 && src.nfi == 2)
-            k_newton_idiff_pcr<4, 2><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
+        bool v4 = g.ncol % 4 == 0 && ca_al16(U) && ca_al16(h->FI[i]);
+        for (int t = 0; t < src.nfull; ++t) v4 = v4 && ca_al16(src.full[t]);
+        for (int t = 0; t < src.nfi; ++t) v4 = v4 && ca_al16(src.fi[t]);
+        if (v4)
+            launch_idiff_pcr<true>(h, src, dtg, i, U, st);
         else
-            k_newton_idiff_pcr<-1, -1><<<nb, 256, 0, st>>>(g, n, src, dtg, U, FI);
+            launch_idiff_pcr<false>(h, src, dtg, i, U, st);
         return check_launch(h, "implicit-diffusion stage");
     }
     if ((s = ensure_bands(h))) return s;

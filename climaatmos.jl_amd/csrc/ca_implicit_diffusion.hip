@@ -329,9 +329,12 @@ __device__ __forceinline__ float solve_f64(double Ld, double Dd, double Ud, doub
 
 }  // namespace idp
 
-template <int NF, int NI>
-__global__ void __launch_bounds__(256) k_newton_idiff_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U,
-                                                          float *FI) {
+// V4: 16-byte row accesses in phases 1 and 3 (ncol a multiple of 4 and every
+// plane 16-byte aligned; the host checks).  Compiled for 3 wavefronts per
+// SIMD (at most 168 VGPRs: 167 and no spill with V4; the 4-byte fallback
+// spills 36-52 bytes).
+template <int NF, int NI, bool V4>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_newton_idiff_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI) {
     constexpr int TC = 16, R = TC + 1;
     __shared__ float tU[5][64][R];
     __shared__ float tF[5][64][R];
@@ -344,6 +347,32 @@ __global__ void __launch_bounds__(256) k_newton_idiff_pcr(DevGeom g, DevNum n, S
 
     // ---- phase 1: U~ = sum cf full + sum ci F_I over all five components
     // (F_I carries u_h here), u_3 zero on the boundary faces
+    if constexpr (V4) {
+        for (int idx = tid; idx < 16 * TC; idx += 256) {
+            const int r = idx / (TC / 4), c4 = 4 * (idx - r * (TC / 4));
+            const int col = col0 + c4;
+            const int64_t q = (int64_t)r * nc + col;
+            const bool ok = col < nc;
+#pragma unroll
+            for (int comp = 0; comp < 5; ++comp) {
+                const bool in = ok && (comp < 4 ? r < nl : (r >= 1 && r <= nl - 1));
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (in) {
+                    const int64_t o = comp * nC + q;
+#pragma unroll
+                    for (int t = 0; t < (NF >= 0 ? NF : CA_MAX_SRC); ++t)
+                        if (t < nf) ca_fma4(src.cf[t], ca_ld4(src.full[t] + o), v);
+#pragma unroll
+                    for (int t = 0; t < (NI >= 0 ? NI : CA_MAX_SRC); ++t)
+                        if (t < ni) ca_fma4(src.ci[t], ca_ld4(src.fi[t] + o), v);
+                }
+                tU[comp][r][c4] = v.x;
+                tU[comp][r][c4 + 1] = v.y;
+                tU[comp][r][c4 + 2] = v.z;
+                tU[comp][r][c4 + 3] = v.w;
+            }
+        }
+    } else
     for (int idx = tid; idx < 64 * TC; idx += 256) {
         const int r = idx / TC, cc = idx - r * TC;
         const int col = col0 + cc;
@@ -589,6 +618,29 @@ __global__ void __launch_bounds__(256) k_newton_idiff_pcr(DevGeom g, DevNum n, S
     __syncthreads();
 
     // ---- phase 3: coalesced stores of U and F_I
+    if constexpr (V4) {
+        for (int idx = tid; idx < 16 * TC; idx += 256) {
+            const int r = idx / (TC / 4), c4 = 4 * (idx - r * (TC / 4));
+            const int col = col0 + c4;
+            if (col >= nc) continue;
+            const int64_t q = (int64_t)r * nc + col;
+            auto row4 = [&](const float (*t)[R]) {
+                return make_float4(t[r][c4], t[r][c4 + 1], t[r][c4 + 2], t[r][c4 + 3]);
+            };
+            if (r < nl) {
+#pragma unroll
+                for (int comp = 0; comp < 4; ++comp) {
+                    *reinterpret_cast<float4 *>(U + comp * nC + q) = row4(tU[comp]);
+                    *reinterpret_cast<float4 *>(FI + comp * nC + q) = row4(tF[comp]);
+                }
+            }
+            if (r <= nl) {
+                *reinterpret_cast<float4 *>(U + 4 * nC + q) = row4(tU[4]);
+                *reinterpret_cast<float4 *>(FI + 4 * nC + q) = row4(tF[4]);
+            }
+        }
+        return;
+    }
     for (int idx = tid; idx < 64 * TC; idx += 256) {
         const int r = idx / TC, cc = idx - r * TC;
         const int col = col0 + cc;
@@ -608,12 +660,15 @@ __global__ void __launch_bounds__(256) k_newton_idiff_pcr(DevGeom g, DevNum n, S
     }
 }
 
-#define CA_INST_IDIFF(NF, NI)                                                                             \
-    template __global__ void k_newton_idiff_pcr<NF, NI>(DevGeom, DevNum, StageSrc, float, float *, float *);
-CA_INST_IDIFF(2, 0)
-CA_INST_IDIFF(3, 1)
-CA_INST_IDIFF(4, 2)
-CA_INST_IDIFF(-1, -1)
+#define CA_INST_IDIFF(NF, NI, V4)                                                                          \
+    template __global__ void k_newton_idiff_pcr<NF, NI, V4>(DevGeom, DevNum, StageSrc, float, float *, float *);
+#define CA_INST_IDIFF_ALL(V4) \
+    CA_INST_IDIFF(2, 0, V4)   \
+    CA_INST_IDIFF(3, 1, V4)   \
+    CA_INST_IDIFF(4, 2, V4)   \
+    CA_INST_IDIFF(-1, -1, V4)
+CA_INST_IDIFF_ALL(false)
+CA_INST_IDIFF_ALL(true)
 
 // U~ = sum cf full + sum ci fi over all five components (the implicit
 // tendencies carry u_h here), u_3 zero on the boundary faces

@@ -53,6 +53,9 @@ def parse():
                     help="run only the ragged column ensemble and print its figures (profiling runs)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the surface-flux and implicit-diffusion step timings")
+    ap.add_argument("--variants-only", action="store_true",
+                    help="only the surface-flux and implicit-diffusion step timings (one JSON line)")
+    ap.add_argument("--variant-steps", type=int, default=30)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
@@ -297,6 +300,12 @@ def main():
     from cadycore.lib import DryDycore
 
     par = P.Params.from_toml()
+    if a.variants_only:
+        zf = G.tanh_stretched_faces(a.z_elem, 30000.0, 30.0)
+        grid = G.cubed_sphere(a.h_elem, zf, par.planet_radius, par.Omega)
+        out = step_variants(torch, grid, par, I.sphere_state(grid, par, seed=2), a.dt, steps=a.variant_steps)
+        print(json.dumps({"workload": f"h_elem={a.h_elem} z_elem={a.z_elem} dt={a.dt:g}s", "step_variants": out}))
+        return 0
     if a.adjoint_only or a.ensemble_only:
         out = adjoint_c4(torch, par) if a.adjoint_only else ensemble_c3(torch, par)
         print(json.dumps(out))
@@ -358,7 +367,7 @@ def main():
 
     if a.copy_calibration:
         copy_calibration(torch)
-    variants = None if a.no_variants else step_variants(torch, grid, par, Y0, a.dt)
+    variants = None if a.no_variants else step_variants(torch, grid, par, Y0, a.dt, steps=a.variant_steps)
     ens = None if a.no_ensemble else ensemble_c3(torch, par)
     adj = None if a.no_adjoint else adjoint_c4(torch, par)
 

@@ -302,27 +302,29 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
 }
 
 ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int32_t every, void *stream) {
-    if (!h || !Y || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    if (!h || !Y || nsteps < 1 || every < 0 || !(dt > 0)) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_record: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode supports nlev <= 128");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no surface fluxes");
     if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no implicit diffusion");
     hipStream_t st = (hipStream_t)stream;
     const int64_t S = h->S;
-    const int nseg = (nsteps + every - 1) / every;
+    const bool all = every == 0;  // the stage records of every step, no checkpoints
+    const int nseg = all ? 0 : (nsteps + every - 1) / every;
     ca_status s;
     h->rec_valid = false;
-    if ((s = adj_ensure(h, every, nseg))) return s;
+    if ((s = adj_ensure(h, all ? nsteps : every, nseg))) return s;
     if ((s = poison_reverse(h, st, true))) return s;
-    if (h->poison && (s = fill_words(h, h->aw_ck, h->aw_ck_n * S, kNaNBits, st))) return s;
+    if (h->poison && h->aw_ck && (s = fill_words(h, h->aw_ck, h->aw_ck_n * S, kNaNBits, st))) return s;
     // the step's initial dss! and filter, then a checkpoint at every segment start
     HIPCHK(h, hipMemcpyAsync(h->U, Y, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
     if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
+    const int64_t RS = rec_size(h);
     for (int t = 0; t < nsteps; ++t) {
-        if (t % every == 0)
+        if (!all && t % every == 0)
             HIPCHK(h, hipMemcpyAsync(h->aw_ck + (int64_t)(t / every) * S, Y, (size_t)S * sizeof(float),
                                      hipMemcpyDeviceToDevice, st));
-        if ((s = one_step(h, Y, dt, st, nullptr))) return s;
+        if ((s = one_step(h, Y, dt, st, all ? h->aw_rec + (int64_t)t * RS : nullptr))) return s;
     }
     h->rec_valid = true;
     h->rec_dt = dt;
@@ -338,13 +340,18 @@ ca_status ca_step_pullback(ca_handle *h, const float *YbN, float *Yb0, float *D0
     const int64_t S = h->S;
     const int nsteps = h->rec_nsteps, every = h->rec_every;
     const double dt = h->rec_dt;
-    const int nseg = (nsteps + every - 1) / every;
+    const bool all = every == 0;
+    const int nseg = all ? 0 : (nsteps + every - 1) / every;
     ca_status s;
-    if ((s = poison_reverse(h, st, true))) return s;
+    // with every step's records kept by the forward, they are the input here
+    if ((s = poison_reverse(h, st, !all))) return s;
     float *Yw = abuf(h, AB_Y);
     if ((s = zero_d0(h, st))) return s;
     HIPCHK(h, hipMemcpyAsync(abuf(h, AB_UB), YbN, (size_t)S * sizeof(float), hipMemcpyDeviceToDevice, st));
     const int64_t RS = rec_size(h);
+    if (all)
+        for (int j = nsteps - 1; j >= 0; --j)
+            if ((s = reverse_step(h, h->aw_rec + (int64_t)j * RS, dt, st))) return s;
     // per segment (last first): re-run it from its checkpoint with stage records, then reverse its steps
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int len = std::min(every, nsteps - sg * every);
@@ -363,14 +370,14 @@ ca_status ca_step_pullback(ca_handle *h, const float *YbN, float *Yb0, float *D0
 
 ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, double dt, int32_t nsteps, int32_t every,
                           float *Yb0, float *D0b, void *stream) {
-    if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 1 || !(dt > 0)) return CA_ERR_ARG;
+    if (!h || !Y0 || !YbN || !Yb0 || nsteps < 1 || every < 0 || !(dt > 0)) return CA_ERR_ARG;
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_step_adjoint: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode supports nlev <= 128");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no surface fluxes");
     if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no implicit diffusion");
     hipStream_t st = (hipStream_t)stream;
     ca_status s;
-    if ((s = adj_ensure(h, every, (nsteps + every - 1) / every))) return s;
+    if ((s = adj_ensure(h, every == 0 ? nsteps : every, every == 0 ? 0 : (nsteps + every - 1) / every))) return s;
     float *Yf = abuf(h, AB_YF);
     HIPCHK(h, hipMemcpyAsync(Yf, Y0, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
     if ((s = ca_step_record(h, Yf, dt, nsteps, every, stream))) return s;

@@ -348,7 +348,10 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
  * ca_step that also keeps what its pullback needs: runs nsteps steps of Y in
  * place (as ca_step) and stores the state at every `every`-th step in the
  * handle (ceil(nsteps / every) states; ca_step_pullback adds `every` steps of
- * stage records, 7 states + 4 Laplacian sets each). */
+ * stage records, 7 states + 4 Laplacian sets each).  every = 0 keeps the
+ * stage records of all nsteps steps instead (nsteps x (7 states + 4 Laplacian
+ * sets) of device memory): the pullback then only reverses, with no
+ * recomputation, and gives bitwise the same cotangents. */
 ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int32_t every, void *stream);
 /* The pullback of the last ca_step_record: given the cotangent YbN of its
  * result, writes the cotangent of its input to Yb0 and, with vertical
@@ -356,7 +359,8 @@ ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int3
  * checkpoint segment is re-run once with its stage states recorded, then
  * reversed.  The column parameters must be those of the recorded run. */
 ca_status ca_step_pullback(ca_handle *h, const float *YbN, float *Yb0, float *D0b, void *stream);
-/* ca_step_record on a copy of Y0 followed by ca_step_pullback.
+/* ca_step_record on a copy of Y0 followed by ca_step_pullback (every as
+ * there, 0 included).
  * The pullback of ca_step(h, Y0, dt, nsteps): given the cotangent YbN of the
  * state after nsteps steps from Y0, writes the cotangent of Y0 to Yb0 and,
  * with vertical diffusion and D0b != NULL, the cotangent of each column's D0

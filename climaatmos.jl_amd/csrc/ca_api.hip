@@ -985,7 +985,9 @@ ca_status ca_ldiv(ca_handle *h, float *dY, const float *R, void *stream) {
     if ((s = poison_work(h, st))) return s;
     {
         Launch l(h, st, KID_HOOKS);
-        if (h->n.idiff)
+        if (h->n.idiff && h->newton_variant >= 1)  // one wavefront per column (nlev + 1 <= 64)
+            k_ldiv_approx_pcr<<<cdiv(g.ncol, 8), 256, 0, st>>>(g, h->B, h->E, h->n.siters, 1.0f, dY, R);
+        else if (h->n.idiff)
             k_ldiv_approx<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, h->E, h->n.siters, 1.0f, dY, R, h->idw);
         else
             k_ldiv<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->B, dY, R, h->scr);

@@ -327,6 +327,91 @@ __device__ __forceinline__ float solve_f64(double Ld, double Dd, double Ud, doub
     return (float)(rd * ca_drcp(Dd));
 }
 
+// Per-lane inputs of the approximate arrowhead solve (lane k = centre k and
+// face k): the centre rows A(rho, u3) = (a0, a1), A(rho e, u3) = (e0, e1),
+// the diffusion blocks A_ee = (elo, ed, eup) and A_hh = (hlo, hd, hup)
+// (identity rows beyond the column), and the scaled residuals b.
+struct Centre {
+    float a0, a1, e0, e1, elo, ed, eup, hlo, hd, hup;
+    float br, bh1, bh2, be;
+};
+struct Sol {
+    float xr, xh1, xh2, xe, xw;
+};
+
+// ApproximateBlockArrowheadIterativeSolve (manual_sparse_jacobian.jl:538-578):
+// a Schur complement reduction on (rho, rho e_tot) with A_11 = diag(-I, A_ee)
+// inverted exactly, then `iters` stationary iterations from x = 0 on the
+// velocity complement S = A_22 - A_21 inv(A_11) A_12, preconditioned by the
+// complement with diag(A_ee) and solved u_h first, then u_3; finally
+// x_1 = inv(A_11) (b_1 - A_12 x_2).  fr is face k's row (identity beyond
+// face nl), bw0 its residual.
+__device__ __forceinline__ Sol arrowhead_solve(int k, int nl, bool hasC, bool intF, bool isF, const Centre &c,
+                                               const FaceRow &fr, float bw0, int iters) {
+    // y_e = inv(A_ee) b_e; b_w' = b_w - A_wrho y_rho - A_we y_e (y_rho = -b_rho)
+    const float ye = solve_f32(c.elo, c.ed, c.eup, c.be);
+    const float pye = pcr::up(ye, 1), pbr = pcr::up(c.br, 1);
+    const float pa0 = pcr::up(c.a0, 1), pa1 = pcr::up(c.a1, 1), pe0 = pcr::up(c.e0, 1), pe1 = pcr::up(c.e1, 1);
+    const float ped = pcr::up(c.ed, 1);
+    float bw = bw0, Pl = fr.lo, Pd = fr.d, Pu = fr.up;
+    if (intF) {
+        bw = bw0 + fr.r0 * pbr + fr.r1 * c.br - (fr.e0 * pye + fr.e1 * ye);
+        Pl = fr.lo + fr.r0 * pa0 - fr.e0 * pe0 / ped;
+        Pd = fr.d + fr.r0 * pa1 + fr.r1 * c.a0 - (fr.e0 * pe1 / ped + fr.e1 * c.e0 / c.ed);
+        Pu = fr.up + fr.r1 * c.a1 - fr.e1 * c.e1 / c.ed;
+    }
+    if (!isF) {
+        Pl = 0.0f;
+        Pd = 1.0f;
+        Pu = 0.0f;
+        bw = 0.0f;
+    }
+    float xh1 = 0.0f, xh2 = 0.0f, xw = 0.0f;
+    for (int it = 0; it < iters; ++it) {
+        float rh1 = c.bh1, rh2 = c.bh2, rw = bw;
+        if (it > 0) {  // r = b' - S x
+            const float x1m = pcr::up(xh1, 1), x1p = pcr::dn(xh1, 1);
+            const float x2m = pcr::up(xh2, 1), x2p = pcr::dn(xh2, 1);
+            const float xwm = pcr::up(xw, 1), xwp = pcr::dn(xw, 1);
+            if (hasC) {
+                rh1 -= c.hd * xh1 + (k >= 1 ? c.hlo * x1m : 0.0f) + (k + 1 < nl ? c.hup * x1p : 0.0f);
+                rh2 -= c.hd * xh2 + (k >= 1 ? c.hlo * x2m : 0.0f) + (k + 1 < nl ? c.hup * x2p : 0.0f);
+            }
+            const float gg = hasC ? c.e0 * xw + c.e1 * xwp : 0.0f;  // A_ew x_w
+            const float mm = solve_f32(c.elo, c.ed, c.eup, gg);
+            const float pmm = pcr::up(mm, 1);
+            const float pA = c.a0 * xw + c.a1 * xwp;  // A_rho,w x_w at centre k
+            const float ppA = pcr::up(pA, 1);
+            float sx = fr.d * xw + (k >= 1 ? fr.lo * xwm : 0.0f) + (k + 1 <= nl ? fr.up * xwp : 0.0f);
+            if (intF) {
+                sx += fr.h0a * x1m + fr.h0b * x2m + fr.h1a * xh1 + fr.h1b * xh2;
+                sx += fr.r0 * ppA + fr.r1 * pA;
+                sx -= fr.e0 * pmm + fr.e1 * mm;
+            }
+            if (isF) rw = bw - sx;
+        }
+        // z_h = inv(A_hh) r_h (two components); z_w = inv(P_ww) (r_w - A_wh z_h)
+        solve2_f32(c.hlo, c.hd, c.hup, rh1, rh2);
+        const float z1m = pcr::up(rh1, 1), z2m = pcr::up(rh2, 1);
+        if (intF) rw -= fr.h0a * z1m + fr.h0b * z2m + fr.h1a * rh1 + fr.h1b * rh2;
+        const float zw = solve_f64(Pl, Pd, Pu, rw);
+        if (hasC) {
+            xh1 += rh1;
+            xh2 += rh2;
+        }
+        if (isF) xw += zw;
+    }
+    const float xwp = pcr::dn(xw, 1);
+    const float gg = hasC ? c.be - (c.e0 * xw + c.e1 * xwp) : 0.0f;
+    Sol x;
+    x.xe = solve_f32(c.elo, c.ed, c.eup, gg);
+    x.xr = c.a0 * xw + c.a1 * xwp - c.br;
+    x.xh1 = xh1;
+    x.xh2 = xh2;
+    x.xw = xw;
+    return x;
+}
+
 }  // namespace idp
 
 // V4: 16-byte row accesses in phases 1 and 3 (ncol a multiple of 4 and every
@@ -512,66 +597,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
             fr.d = -dtg * beta_w(g, n, k, col) - 1.0f;
         }
         // ---- ApproximateBlockArrowheadIterativeSolve
-        // y_e = inv(A_ee) b_e; b_w' = b_w - A_wrho y_rho - A_we y_e (y_rho = -b_rho)
-        const float ye = idp::solve_f32(hasC ? elo : 0.0f, hasC ? ed : 1.0f, hasC ? eup : 0.0f, hasC ? Re : 0.0f);
-        const float pye = pcr::up(ye, 1), pRr = pcr::up(Rr, 1);
-        const float pa0 = pcr::up(a0, 1), pa1 = pcr::up(a1, 1), pe0 = pcr::up(e0, 1), pe1 = pcr::up(e1, 1);
-        const float ped = pcr::up(ed, 1);
-        float bw = Ru3, Pl = fr.lo, Pd = fr.d, Pu = fr.up;
-        if (intF) {
-            bw = Ru3 + fr.r0 * pRr + fr.r1 * Rr - (fr.e0 * pye + fr.e1 * ye);
-            Pl = fr.lo + fr.r0 * pa0 - fr.e0 * pe0 / ped;
-            Pd = fr.d + fr.r0 * pa1 + fr.r1 * a0 - (fr.e0 * pe1 / ped + fr.e1 * e0 / ed);
-            Pu = fr.up + fr.r1 * a1 - fr.e1 * e1 / ed;
-        }
-        if (!isF) {
-            Pl = 0.0f;
-            Pd = 1.0f;
-            Pu = 0.0f;
-            bw = 0.0f;
-        }
-        // stationary iterations from x = 0, preconditioned by the complement
-        // with diag(A_ee): u_h first, then u_3
-        float xh1 = 0.0f, xh2 = 0.0f, xw = 0.0f;
-        for (int it = 0; it < n.siters; ++it) {
-            float rh1 = hasC ? Rh1 : 0.0f, rh2 = hasC ? Rh2 : 0.0f, rw = bw;
-            if (it > 0) {  // r = b' - S x
-                const float x1m = pcr::up(xh1, 1), x1p = pcr::dn(xh1, 1);
-                const float x2m = pcr::up(xh2, 1), x2p = pcr::dn(xh2, 1);
-                const float xwm = pcr::up(xw, 1), xwp = pcr::dn(xw, 1);
-                if (hasC) {
-                    rh1 -= hd * xh1 + (k >= 1 ? hlo * x1m : 0.0f) + (k + 1 < nl ? hup * x1p : 0.0f);
-                    rh2 -= hd * xh2 + (k >= 1 ? hlo * x2m : 0.0f) + (k + 1 < nl ? hup * x2p : 0.0f);
-                }
-                const float gg = hasC ? e0 * xw + e1 * xwp : 0.0f;  // A_ew x_w
-                const float mm = idp::solve_f32(hasC ? elo : 0.0f, hasC ? ed : 1.0f, hasC ? eup : 0.0f, gg);
-                const float pmm = pcr::up(mm, 1);
-                const float pA = a0 * xw + a1 * xwp;  // A_rho,w x_w at centre k
-                const float ppA = pcr::up(pA, 1);
-                float sx = fr.d * xw + (k >= 1 ? fr.lo * xwm : 0.0f) + (k + 1 <= nl ? fr.up * xwp : 0.0f);
-                if (intF) {
-                    sx += fr.h0a * x1m + fr.h0b * x2m + fr.h1a * xh1 + fr.h1b * xh2;
-                    sx += fr.r0 * ppA + fr.r1 * pA;
-                    sx -= fr.e0 * pmm + fr.e1 * mm;
-                }
-                if (isF) rw = bw - sx;
-            }
-            // z_h = inv(A_hh) r_h (two components); z_w = inv(P_ww) (r_w - A_wh z_h)
-            idp::solve2_f32(hasC ? hlo : 0.0f, hasC ? hd : 1.0f, hasC ? hup : 0.0f, rh1, rh2);
-            const float z1m = pcr::up(rh1, 1), z2m = pcr::up(rh2, 1);
-            if (intF) rw -= fr.h0a * z1m + fr.h0b * z2m + fr.h1a * rh1 + fr.h1b * rh2;
-            const float zw = idp::solve_f64(Pl, Pd, Pu, rw);
-            if (hasC) {
-                xh1 += rh1;
-                xh2 += rh2;
-            }
-            if (isF) xw += zw;
-        }
-        // x_1 = inv(A_11) (b_1 - A_12 x_2)
-        const float xwp = pcr::dn(xw, 1);
-        const float gg = hasC ? Re - (e0 * xw + e1 * xwp) : 0.0f;
-        const float xe = idp::solve_f32(hasC ? elo : 0.0f, hasC ? ed : 1.0f, hasC ? eup : 0.0f, gg);
-        const float xr = a0 * xw + a1 * xwp - Rr;
+        idp::Centre cb;
+        cb.a0 = a0;
+        cb.a1 = a1;
+        cb.e0 = e0;
+        cb.e1 = e1;
+        cb.elo = hasC ? elo : 0.0f;
+        cb.ed = hasC ? ed : 1.0f;
+        cb.eup = hasC ? eup : 0.0f;
+        cb.hlo = hasC ? hlo : 0.0f;
+        cb.hd = hasC ? hd : 1.0f;
+        cb.hup = hasC ? hup : 0.0f;
+        cb.br = Rr;
+        cb.be = hasC ? Re : 0.0f;
+        cb.bh1 = hasC ? Rh1 : 0.0f;
+        cb.bh2 = hasC ? Rh2 : 0.0f;
+        const idp::Sol x = idp::arrowhead_solve(k, nl, hasC, intF, isF, cb, fr, Ru3, n.siters);
+        const float xh1 = x.xh1, xh2 = x.xh2, xe = x.xe, xw = x.xw, xr = x.xr;
         // Newton update
         const float rho_n = rho - xr;
         const float u1_n = u1 - xh1, u2_n = u2 - xh2;
@@ -669,6 +711,92 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
     CA_INST_IDIFF(-1, -1, V4)
 CA_INST_IDIFF_ALL(false)
 CA_INST_IDIFF_ALL(true)
+
+// ldiv! with implicit diffusion from the stored bands (the hook entry point
+// ca_ldiv after ca_update_jacobian): dY = solve(A, rscale R) with the
+// arrowhead solve of the fused stage, one wavefront per column.  A workgroup
+// stages 8 columns of the 26 band and residual planes through LDS with
+// coalesced row loads; the results leave through the residual tile.
+__global__ void __launch_bounds__(256) k_ldiv_approx_pcr(DevGeom g, Bands A, DiffBands E, int iters, float rscale,
+                                                         float *dY, const float *R) {
+    constexpr int TC = 8, RW = TC + 1;
+    // planes: 0-3 centre rows (Aru0, Aru1, Aeu0, Aeu1); 4-14 face rows (Aur0, Aur1,
+    // Aue0, Aue1, Auh0a, Auh0b, Auh1a, Auh1b, lo, d, up); 15-20 E (elo, ed, eup,
+    // hlo, hd, hup); 21-25 R (rho, u_1, u_2, rho e_tot, u_3), then dY in place
+    constexpr int NP = 26;
+    __shared__ float t[NP][64][RW];
+    const int nl = g.nlev;
+    const int64_t nC = g.nC, nc = g.ncol;
+    const int col0 = ca_xcd_tile(blockIdx.x, gridDim.x) * TC;
+    const int tid = threadIdx.x;
+    const float *src[NP] = {A.Aru0, A.Aru1, A.Aeu0, A.Aeu1, A.Aur0, A.Aur1, A.Aue0, A.Aue1, A.Auh0a,
+                            A.Auh0b, A.Auh1a, A.Auh1b, A.lo, A.d, A.up, E.elo, E.ed, E.eup, E.hlo,
+                            E.hd, E.hup, R, R + nC, R + 2 * nC, R + 3 * nC, R + 4 * nC};
+    for (int idx = tid; idx < 64 * TC; idx += 256) {
+        const int r = idx / TC, cc = idx - r * TC;
+        const int col = col0 + cc;
+        const int64_t q = (int64_t)r * nc + col;
+        const bool ok = col < nc, cen = r < nl, face = r <= nl;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const bool fp = (p >= 4 && p <= 14) || p == 25;
+            t[p][r][cc] = ok && (fp ? face : cen) ? src[p][q] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int k = tid & 63, wid = tid >> 6;
+    const bool hasC = k < nl, intF = k >= 1 && k <= nl - 1, isF = k <= nl;
+    for (int cc = wid; cc < TC; cc += 4) {
+        if (col0 + cc >= nc) break;  // uniform across the wavefront
+        idp::Centre c;
+        c.a0 = t[0][k][cc];
+        c.a1 = t[1][k][cc];
+        c.e0 = t[2][k][cc];
+        c.e1 = t[3][k][cc];
+        c.elo = hasC ? t[15][k][cc] : 0.0f;
+        c.ed = hasC ? t[16][k][cc] : 1.0f;
+        c.eup = hasC ? t[17][k][cc] : 0.0f;
+        c.hlo = hasC ? t[18][k][cc] : 0.0f;
+        c.hd = hasC ? t[19][k][cc] : 1.0f;
+        c.hup = hasC ? t[20][k][cc] : 0.0f;
+        c.br = rscale * t[21][k][cc];
+        c.bh1 = rscale * t[22][k][cc];
+        c.bh2 = rscale * t[23][k][cc];
+        c.be = rscale * t[24][k][cc];
+        FaceRow fr;
+        fr.r0 = t[4][k][cc];
+        fr.r1 = t[5][k][cc];
+        fr.e0 = t[6][k][cc];
+        fr.e1 = t[7][k][cc];
+        fr.h0a = t[8][k][cc];
+        fr.h0b = t[9][k][cc];
+        fr.h1a = t[10][k][cc];
+        fr.h1b = t[11][k][cc];
+        fr.lo = t[12][k][cc];
+        fr.d = isF ? t[13][k][cc] : 1.0f;
+        fr.up = t[14][k][cc];
+        const float bw = rscale * t[25][k][cc];
+        const idp::Sol x = idp::arrowhead_solve(k, nl, hasC, intF, isF, c, fr, bw, iters);
+        if (hasC) {
+            t[21][k][cc] = x.xr;
+            t[22][k][cc] = x.xh1;
+            t[23][k][cc] = x.xh2;
+            t[24][k][cc] = x.xe;
+        }
+        if (isF) t[25][k][cc] = x.xw;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 64 * TC; idx += 256) {
+        const int r = idx / TC, cc = idx - r * TC;
+        const int col = col0 + cc;
+        if (col >= nc) continue;
+        const int64_t q = (int64_t)r * nc + col;
+        if (r < nl)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) dY[p * nC + q] = t[21 + p][r][cc];
+        if (r <= nl) dY[4 * nC + q] = t[25][r][cc];
+    }
+}
 
 // U~ = sum cf full + sum ci fi over all five components (the implicit
 // tendencies carry u_h here), u_3 zero on the boundary faces

@@ -52,6 +52,8 @@ __global__ void k_update_jacobian(DevGeom g, DevNum n, const float *Y, float dtg
 __global__ void k_ldiv(DevGeom g, Bands B, float *dY, const float *R, float *scr);
 __global__ void k_correct(DevGeom g, DevNum n, float *Yt, const float *Y);
 __global__ void k_newton_stage(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI, float *scr);
+__global__ void k_ldiv_approx_pcr(DevGeom g, Bands A, DiffBands E, int iters, float rscale, float *dY,
+                                  const float *R);
 template <int NF, int NI, bool V4>
 __global__ void k_newton_idiff_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI);
 template <int TC, int NF, int NI, bool V4>

@@ -87,6 +87,9 @@ class ColumnEnsemble:
         c.rayleigh = int(bool(numerics.rayleigh))
         c.zd_rayleigh, c.alpha_uh, c.alpha_w = params.zd_rayleigh, params.alpha_rayleigh_uh, params.alpha_rayleigh_w
         c.vert_diff = int(bool(numerics.vert_diff))
+        c.vert_diff_model = numerics.vert_diff_kind()
+        c.vert_diff_H = float(numerics.vert_diff_H)
+        c.vert_diff_C_E = float(numerics.vert_diff_C_E)
         c.dt = numerics.dt
         h = C.c_void_p()
         st = self.lib.ca_ensemble_create(C.byref(p), C.byref(c), len(self.members), arr, C.byref(h))

@@ -56,6 +56,7 @@ class CaConfig(C.Structure):
         ("sfc_flux", C.c_int32), ("Cd", C.c_double), ("Ch", C.c_double), ("gustiness", C.c_double),
         ("T_sfc_col", C.POINTER(C.c_double)),
         ("implicit_diffusion", C.c_int32), ("solve_iters", C.c_int32),
+        ("vert_diff_model", C.c_int32), ("vert_diff_H", C.c_double), ("vert_diff_C_E", C.c_double),
     ]
 
 
@@ -206,7 +207,10 @@ def create_args(grid: Grid, params: Params, numerics: Numerics, alpha_w_col=None
     if alpha_w_col is not None:
         c.alpha_w_col = _dp(arr("aw", alpha_w_col))
     c.vert_diff = int(bool(numerics.vert_diff))
-    if numerics.vert_diff:
+    c.vert_diff_model = numerics.vert_diff_kind()
+    c.vert_diff_H = float(numerics.vert_diff_H)
+    c.vert_diff_C_E = float(numerics.vert_diff_C_E)
+    if numerics.vert_diff and c.vert_diff_model == 0:
         if D0_col is None:
             raise ValueError("vertical diffusion needs D0_col")
         c.D0_col = _dp(arr("D0", D0_col))

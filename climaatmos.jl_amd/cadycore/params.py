@@ -111,10 +111,20 @@ class Numerics:
     Cd: float = 0.0044
     Ch: float = 0.0044
     gustiness: float = 1.0
+    # the vertical-diffusion closure (vert_diff, src/config/model_getters.jl:341-355):
+    # "DecayWithHeightDiffusion" (K = D0_col exp(-z / H); H = inf keeps K = D0_col)
+    # or "VerticalDiffusion" (K = C_E |u_h(lowest centre)| dz_1 / 2, tapered above
+    # 850 hPa; precomputed_quantities.jl:651-676)
+    vert_diff_model: str = "DecayWithHeightDiffusion"
+    vert_diff_H: float = math.inf
+    vert_diff_C_E: float = 0.044
 
     @staticmethod
     def upwinding_code(name: str) -> int:
         return _UPWIND_NAMES[name]
+
+    def vert_diff_kind(self) -> int:
+        return {"DecayWithHeightDiffusion": 0, "VerticalDiffusion": 1}[self.vert_diff_model]
 
     def nu4(self, h: float):
         nu_v = self.c_nu * h**3

@@ -61,6 +61,7 @@ struct ca_handle {
     std::vector<ProfRec> recs;
     double aE[4][4], aI[4][4], b[4];
     int newton_variant = 1;
+    bool vd_plain = true;  // no vertical diffusion, or DecayWithHeightDiffusion with H = inf (the reverse mode's)
     int explicit_chunk = 8;
     int dss_chunk = 8;
     int lap_chunk = 4;  // levels per work item of k_laplacians
@@ -527,10 +528,23 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
         if ((s = upload(h, &p, tofloat(cfg->alpha_w_col, nc)))) return s;
         g.aw_col = p;
     }
-    if (cfg->vert_diff) {
+    g.kdec = nullptr;
+    if (cfg->vert_diff && cfg->D0_col) {
         float *p = nullptr;
         if ((s = upload(h, &p, tofloat(cfg->D0_col, nc)))) return s;
         g.D0_col = p;
+    }
+    if (cfg->vert_diff) {
+        // DecayWithHeightDiffusion's exp(-(z_c - z_sfc) / H) per level (1 for H = inf)
+        std::vector<float> dec(nl);
+        for (int k = 0; k < nl; ++k) {
+            const double zc = 0.5 * (geom->z_face[k] + geom->z_face[k + 1]);
+            const double H = cfg->vert_diff_H;
+            dec[k] = H > 0 && std::isfinite(H) ? (float)std::exp(-(zc - geom->z_face[0]) / H) : 1.0f;
+        }
+        float *p = nullptr;
+        if ((s = upload(h, &p, dec))) return s;
+        g.kdec = p;
     }
     // DSS tables (gather rows per column; element clusters on the sphere)
     {
@@ -633,6 +647,11 @@ ca_status create_impl(const ca_geometry *geom, const ca_params *par, const ca_co
     n.sflux = cfg->sfc_flux != 0;
     n.idiff = cfg->implicit_diffusion && cfg->vert_diff;
     n.siters = cfg->solve_iters > 0 ? cfg->solve_iters : 1;
+    n.vdk = cfg->vert_diff_model;
+    h->vd_plain = !cfg->vert_diff ||
+                  (cfg->vert_diff_model == 0 && !(cfg->vert_diff_H > 0 && std::isfinite(cfg->vert_diff_H)));
+    n.CE = (float)cfg->vert_diff_C_E;
+    n.za = (float)(0.5 * (geom->z_face[1] - geom->z_face[0]));
     n.Cd = (float)cfg->Cd;
     n.Ch = (float)cfg->Ch;
     n.gust = (float)cfg->gustiness;
@@ -734,7 +753,14 @@ std::string check_create_args(const ca_geometry *geom, const ca_params *par, con
     if (!is_gll4(geom->gll_w, geom->gll_D)) return "gll_w / gll_D must be the Nq = 4 GLL weights and derivative matrix";
     if (cfg->upwinding != 0 && cfg->upwinding != 1 && cfg->upwinding != 3)
         return "upwinding must be 0 (none), 1 (first_order) or 3 (vanleer_limiter)";
-    if (cfg->vert_diff && !cfg->D0_col) return "vertical diffusion needs D0_col";
+    if (cfg->vert_diff && cfg->vert_diff_model != 0 && cfg->vert_diff_model != 1)
+        return "vert_diff_model must be 0 (DecayWithHeightDiffusion) or 1 (VerticalDiffusion)";
+    if (cfg->vert_diff && cfg->vert_diff_model == 0 && !cfg->D0_col)
+        return "DecayWithHeightDiffusion needs D0_col";
+    if (cfg->vert_diff && cfg->vert_diff_model == 0 && !(cfg->vert_diff_H >= 0))
+        return "DecayWithHeightDiffusion needs vert_diff_H > 0 (0 or INFINITY: no decay)";
+    if (cfg->vert_diff && cfg->vert_diff_model == 1 && !(cfg->vert_diff_C_E >= 0 && std::isfinite(cfg->vert_diff_C_E)))
+        return "VerticalDiffusion needs a finite vert_diff_C_E >= 0";
     if (cfg->sfc_flux && !cfg->T_sfc_col) return "surface fluxes need T_sfc_col";
     if (cfg->implicit_diffusion && !cfg->vert_diff) return "implicit diffusion needs vertical diffusion";
     if (cfg->implicit_diffusion && (cfg->solve_iters < 1 || cfg->solve_iters > 100))

@@ -58,6 +58,7 @@ struct DevGeom {
     const ushort4 *dl_slot;
     const float4 *dl_w, *dl_M;
     const float *aw_col, *D0_col;        // optional per-column sponge alpha_w / diffusivity
+    const float *kdec;                   // [nlev] exp(-(z_c - z_sfc) / H) of DecayWithHeightDiffusion
     const float *Tsfc;                   // per-column surface temperature (surface fluxes)
 };
 
@@ -71,7 +72,30 @@ struct DevNum {
     float Cd, Ch, gust;  // surface flux exchange coefficients and wind-speed floor
     int hyper, upw, rayleigh, vdiff, sphere, sflux;
     int idiff, siters;  // vertical diffusion implicit; iterations of its linear solve
+    int vdk;            // closure: 0 DecayWithHeightDiffusion, 1 VerticalDiffusion
+    float CE, za;       // VerticalDiffusion's C_E and half the lowest layer's depth
 };
+
+// ------------------------------------------------------------- eddy diffusivity
+// K_h of the centre at level k of column col (ᶜcompute_eddy_diffusivity_coefficient,
+// cache/eddy_diffusivity_coefficient.jl:16-41; precomputed_quantities.jl:651-676):
+// DecayWithHeightDiffusion D0 exp(-(z - z_sfc) / H), or VerticalDiffusion
+// C_E |u_h| z_a from the lowest centre's wind speed spd0, tapered by
+// exp(-((85000 - p) / 10000)^2) above 850 hPa.
+__device__ __forceinline__ float ca_Kc(const DevGeom &g, const DevNum &n, int col, int k, float p, float spd0) {
+    if (n.vdk == 0) return g.D0_col[col] * g.kdec[k];
+    const float KE = n.CE * spd0 * n.za;
+    if (p > 85000.0f) return KE;
+    const float x = (85000.0f - p) * 1.0e-4f;
+    return KE * expf(-(x * x));
+}
+// rho K on a face: interp(rho) / interp(1 / max(K, eps)), the harmonic mean
+// of the two centre values (vertical_diffusion_boundary_layer.jl:86-90)
+__device__ __forceinline__ float ca_rhoK(float ra, float rb, float Ka, float Kb) {
+    const float e = 1.1920929e-07f;
+    const float Kf = 1.0f / (0.5f * (1.0f / fmaxf(Ka, e) + 1.0f / fmaxf(Kb, e)));
+    return (0.5f * (ra + rb)) * Kf;
+}
 
 // ------------------------------------------------------------- launch order
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD

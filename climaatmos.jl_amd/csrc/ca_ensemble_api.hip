@@ -289,6 +289,11 @@ ca_status ca_ensemble_create(const ca_params *params, const ca_config *config, i
         g_create_error = "ca_ensemble_create: the column ensembles have no implicit diffusion";
         return CA_ERR_ARG;
     }
+    if (config->vert_diff && (config->vert_diff_model != 0 ||
+                              (config->vert_diff_H > 0 && std::isfinite(config->vert_diff_H)))) {
+        g_create_error = "ca_ensemble_create: the column ensembles take DecayWithHeightDiffusion with H = inf only";
+        return CA_ERR_ARG;
+    }
     ca_ensemble *e = new ca_ensemble();
     ca_status s = ens_create_impl(e, params, config, nmembers, members);
     if (s == CA_OK) s = resolve_noalloc_scratch(e);

@@ -27,11 +27,9 @@
 // s_d flux (J_f g^33-weighted) and the Cartesian momentum fluxes, as the
 // explicit tendency forms them (k_explicit, VDIFF)
 __device__ __forceinline__ void vdiff_face(const DevGeom &g, const DevNum &n, int col, int f, const CV &a, const CV &b,
-                                           float &Fs, float &F0, float &F1, float &F2) {
+                                           float Ka, float Kb, float &Fs, float &F0, float &F1, float &F2) {
     const int64_t nc = g.ncol;
-    const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
-    const float Kf = 1.0f / (0.5f * (1.0f / Kh + 1.0f / Kh));
-    const float rK = (0.5f * (a.rho + b.rho)) * Kf;
+    const float rK = ca_rhoK(a.rho, b.rho, Ka, Kb);
     const float sf = g.sf[f];
     const float Jf = g.Jh[col] * sf * sf * g.Jvf[f];
     const float iJv = g.iJvf[f];
@@ -64,6 +62,16 @@ __device__ __forceinline__ void vdiff_face(const DevGeom &g, const DevNum &n, in
     F2 = fac * du[2] * iJv;
 }
 
+// the lowest centre's horizontal wind speed |u_h| = sqrt(u_i u^i) (VerticalDiffusion's
+// closure; 0 for DecayWithHeightDiffusion, which does not read it)
+__device__ __forceinline__ float spd_lowest(const DevGeom &g, const DevNum &n, const float *Y, int col) {
+    if (n.vdk != 1) return 0.0f;
+    const float u1 = Y[g.nC + col], u2 = Y[2 * g.nC + col];
+    float c1, c2;
+    ca_contra(g, col, g.is2c[0], u1, u2, c1, c2);
+    return sqrtf(u1 * c1 + u2 * c2);
+}
+
 // Yt += the vertical diffusion tendency (rho e_tot, u_h) at centre l of every column
 __global__ void __launch_bounds__(128) k_vdiff_add(DevGeom g, DevNum n, const float *Y, float *Yt) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
@@ -71,14 +79,17 @@ __global__ void __launch_bounds__(128) k_vdiff_add(DevGeom g, DevNum n, const fl
     const int nl = g.nlev;
     const int64_t nC = g.nC, nc = g.ncol;
     const CV c = cv_load(g, n, Y, l, col);
+    const float spd0 = spd_lowest(g, n, Y, col);
+    const float Kc = ca_Kc(g, n, col, l, c.p, spd0);
     float Fs[2] = {0.0f, 0.0f}, Fu[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
     if (l >= 1) {
         const CV a = cv_load(g, n, Y, l - 1, col);
-        vdiff_face(g, n, col, l, a, c, Fs[0], Fu[0][0], Fu[0][1], Fu[0][2]);
+        vdiff_face(g, n, col, l, a, c, ca_Kc(g, n, col, l - 1, a.p, spd0), Kc, Fs[0], Fu[0][0], Fu[0][1], Fu[0][2]);
     }
     if (l + 1 <= nl - 1) {
         const CV b = cv_load(g, n, Y, l + 1, col);
-        vdiff_face(g, n, col, l + 1, c, b, Fs[1], Fu[1][0], Fu[1][1], Fu[1][2]);
+        vdiff_face(g, n, col, l + 1, c, b, Kc, ca_Kc(g, n, col, l + 1, b.p, spd0), Fs[1], Fu[1][0], Fu[1][1],
+                   Fu[1][2]);
     }
     const float iJc = g.iJh[col] * g.isJc[l];
     const int64_t q = (int64_t)l * nc + col;
@@ -106,21 +117,23 @@ __global__ void __launch_bounds__(128) k_diffusion_jacobian(DevGeom g, DevNum n,
     const int nl = g.nlev;
     const int64_t nc = g.ncol;
     const float *rho = Y;
-    const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
-    const float Kf = 1.0f / (0.5f * (1.0f / Kh + 1.0f / Kh));
     const int64_t q = (int64_t)l * nc + col;
     const float r0 = rho[q];
     const float Jc = g.Jh[col] * g.sc[l] * g.sc[l] * g.Jvc[l];
+    // centre diffusivities (VerticalDiffusion's depend on the pressure)
+    const float spd0 = spd_lowest(g, n, Y, col);
+    auto Kat = [&](int k) { return ca_Kc(g, n, col, k, n.vdk == 1 ? cv_load(g, n, Y, k, col).p : 0.0f, spd0); };
+    const float K0 = Kat(l);
     float dlo = 0.0f, dup = 0.0f, rm = 1.0f, rp = 1.0f;
     if (l >= 1) {
         rm = rho[q - nc];
         const float sf = g.sf[l];
-        dlo = (g.Jh[col] * sf * sf * g.Jvf[l]) * g.g33f[l] * ((0.5f * (rm + r0)) * Kf) / Jc;
+        dlo = (g.Jh[col] * sf * sf * g.Jvf[l]) * g.g33f[l] * ca_rhoK(rm, r0, Kat(l - 1), K0) / Jc;
     }
     if (l + 1 <= nl - 1) {
         rp = rho[q + nc];
         const float sf = g.sf[l + 1];
-        dup = (g.Jh[col] * sf * sf * g.Jvf[l + 1]) * g.g33f[l + 1] * ((0.5f * (r0 + rp)) * Kf) / Jc;
+        dup = (g.Jh[col] * sf * sf * g.Jvf[l + 1]) * g.g33f[l + 1] * ca_rhoK(r0, rp, K0, Kat(l + 1)) / Jc;
     }
     const float dd = -(dlo + dup);
     const float kc = n.cp_d / n.cv_d;
@@ -539,8 +552,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         a.ct1 = pcr::up(c.ct1, 1);
         a.ct2 = pcr::up(c.ct2, 1);
         // vertical diffusion: fluxes through face k, the tendency of centre k
+        // the centre's diffusivity, and the lowest centre's wind speed (lane 0)
+        const float spd0 = n.vdk == 1 ? __shfl(sqrtf(c.u1 * c.ct1 + c.u2 * c.ct2), 0, 64) : 0.0f;
+        const float Kd = ca_Kc(g, n, col, kc, c.p, spd0);
+        const float Kdm = pcr::up(Kd, 1), Kdp = pcr::dn(Kd, 1);
         float Fs = 0.0f, Fu0 = 0.0f, Fu1 = 0.0f, Fu2 = 0.0f;
-        if (intF) vdiff_face(g, n, col, k, a, c, Fs, Fu0, Fu1, Fu2);
+        if (intF) vdiff_face(g, n, col, k, a, c, Kdm, Kd, Fs, Fu0, Fu1, Fu2);
         const float Fs1 = pcr::dn(Fs, 1), Fu01 = pcr::dn(Fu0, 1), Fu11 = pcr::dn(Fu1, 1), Fu21 = pcr::dn(Fu2, 1);
         float Rh1 = 0.0f, Rh2 = 0.0f;
         if (hasC) {
@@ -564,18 +581,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         // diffusion blocks of centre k (rows >= nl: identity)
         float elo = 0.0f, ed = 1.0f, eup = 0.0f, hlo = 0.0f, hd = 1.0f, hup = 0.0f;
         if (hasC) {
-            const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
-            const float Kf = 1.0f / (0.5f * (1.0f / Kh + 1.0f / Kh));
             const float rm = a.rho, rp = pcr::dn(c.rho, 1), r0 = c.rho;
             const float Jc = c.J;
             float dlo = 0.0f, dup = 0.0f;
             if (k >= 1) {
                 const float sf = g.sf[k];
-                dlo = (g.Jh[col] * sf * sf * g.Jvf[k]) * g.g33f[k] * ((0.5f * (rm + r0)) * Kf) / Jc;
+                dlo = (g.Jh[col] * sf * sf * g.Jvf[k]) * g.g33f[k] * ca_rhoK(rm, r0, Kdm, Kd) / Jc;
             }
             if (k + 1 <= nl - 1) {
                 const float sf = g.sf[k + 1];
-                dup = (g.Jh[col] * sf * sf * g.Jvf[k + 1]) * g.g33f[k + 1] * ((0.5f * (r0 + rp)) * Kf) / Jc;
+                dup = (g.Jh[col] * sf * sf * g.Jvf[k + 1]) * g.g33f[k + 1] * ca_rhoK(r0, rp, Kd, Kdp) / Jc;
             }
             const float dd = -(dlo + dup);
             elo = k >= 1 ? dtg * dlo * (kcv / rm) : 0.0f;

@@ -61,6 +61,7 @@ struct ColState {
     float dr, d1, d2, de;      // partial centre tendencies
     float sd;                  // dry static energy (vertical diffusion)
     float uc0, uc1, uc2;       // velocity in the diffusion basis
+    float Kd;                  // eddy diffusivity K_h of the centre
 };
 
 // Work item = (node, chunk of levels): chunk c owns centres and faces
@@ -91,14 +92,16 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
     const float f3n = g.f3n[col], f12a = g.f12a[col], f12b = g.f12b[col];
     const ColMet cm = ca_colmet(g, col);
     const uint32_t cb = (uint32_t)col * 4u;  // the lane's byte offset in every level row
-    float Kf = 0.0f;  // face diffusivity: harmonic mean of the (equal) centre values
-    if (VDIFF) {
-        const float Kh = fmaxf(g.D0_col[col], 1.1920929e-07f);
-        Kf = 1.0f / (0.5f * (1.0f / Kh + 1.0f / Kh));
+    float spd0 = 0.0f;  // the lowest centre's wind speed (VerticalDiffusion's closure)
+    if (VDIFF && n.vdk == 1) {
+        const float a1 = ca_ldo(Y + nC, cb), a2 = ca_ldo(Y + 2 * nC, cb);
+        float c1, c2;
+        ca_contra(cm, g.is2c[0], a1, a2, c1, c2);
+        spd0 = sqrtf(a1 * c1 + a2 * c2);
     }
     ColState prev, cur;
     prev.rhoJ = prev.irhoJ = prev.Kh = prev.Kv = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
-    prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = 0.0f;
+    prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = prev.Kd = 0.0f;
     float Gp1 = 0.0f, Gp2 = 0.0f;      // G at the face below the current centre
     float Fs_prev = 0.0f, Fu0_prev = 0.0f, Fu1_prev = 0.0f, Fu2_prev = 0.0f;  // diffusive fluxes
     const int k0 = chunk > 0 ? (int)blockIdx.y * chunk : 0;
@@ -235,6 +238,7 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
             cur.de = de;
             if (VDIFF) {
                 cur.sd = n.cp_d * (T - n.T_0) + g.Phic[k];
+                cur.Kd = ca_Kc(g, n, col, k, p, spd0);
                 if (sph) {
                     const float *B = g.basis;
                     float is = ca_rcp(s);
@@ -278,8 +282,8 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
                     du3 -= n.nu_v * ((0.5f * (wA * prev.l4 + wB * cur.l4)) * ca_rcp(0.5f * (wA + wB)));
                 }
                 if (VDIFF) {
-                    // vertical_diffusion_boundary_layer.jl:90-103 with K_h = D0 (H = Inf)
-                    float rK = (0.5f * (prev.rho + cur.rho)) * Kf;
+                    // vertical_diffusion_boundary_layer.jl:86-103
+                    float rK = ca_rhoK(prev.rho, cur.rho, prev.Kd, cur.Kd);
                     float iJv = g.iJvf[k];
                     Fs = Jf * (iJv * iJv * (-(rK * (cur.sd - prev.sd))));
                     float fac = Jf * iJv * (-2.0f * rK) * 0.5f;

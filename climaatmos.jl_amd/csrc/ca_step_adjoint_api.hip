@@ -278,6 +278,8 @@ ca_status ca_remaining_tendency_vjp(ca_handle *h, float *Yb, const float *Ytb, c
     if (!h->n.sphere) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: sphere grids only (columns: ca_ensemble_adjoint)");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: the reverse mode has no surface fluxes");
     if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: the reverse mode has no implicit diffusion");
+    if (!h->vd_plain)
+        return fail(h, CA_ERR_ARG, "ca_remaining_tendency_vjp: the reverse mode takes DecayWithHeightDiffusion with H = inf only");
     hipStream_t st = (hipStream_t)stream;
     const DevGeom &g = h->g;
     ca_status s;
@@ -307,6 +309,8 @@ ca_status ca_step_record(ca_handle *h, float *Y, double dt, int32_t nsteps, int3
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode supports nlev <= 128");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no surface fluxes");
     if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode has no implicit diffusion");
+    if (!h->vd_plain)
+        return fail(h, CA_ERR_ARG, "ca_step_record: the reverse mode takes DecayWithHeightDiffusion with H = inf only");
     hipStream_t st = (hipStream_t)stream;
     const int64_t S = h->S;
     const bool all = every == 0;  // the stage records of every step, no checkpoints
@@ -375,6 +379,8 @@ ca_status ca_step_adjoint(ca_handle *h, const float *Y0, const float *YbN, doubl
     if (h->g.nlev > kMaxAdjLevels) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode supports nlev <= 128");
     if (h->n.sflux) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no surface fluxes");
     if (h->n.idiff) return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode has no implicit diffusion");
+    if (!h->vd_plain)
+        return fail(h, CA_ERR_ARG, "ca_step_adjoint: the reverse mode takes DecayWithHeightDiffusion with H = inf only");
     hipStream_t st = (hipStream_t)stream;
     ca_status s;
     if ((s = adj_ensure(h, every == 0 ? nsteps : every, every == 0 ? 0 : (nsteps + every - 1) / every))) return s;

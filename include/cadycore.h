@@ -111,6 +111,19 @@ typedef struct ca_config {
      * vert_diff; the reverse mode and the column ensembles return CA_ERR_ARG. */
     int32_t implicit_diffusion;
     int32_t solve_iters;    /* approximate_linear_solve_iters, 1 .. 100 (default 1) */
+    /* The vertical-diffusion closure (vert_diff, src/config/model_getters.jl:341-355;
+     * ᶜcompute_eddy_diffusivity_coefficient, cache/eddy_diffusivity_coefficient.jl:16-41):
+     *   0 DecayWithHeightDiffusion: K = D0_col exp(-(z - z_sfc) / vert_diff_H)
+     *     (vert_diff_H = 0 or INFINITY: K = D0_col, the zero-initialised default);
+     *   1 VerticalDiffusion: K = vert_diff_C_E |u_h| dz_1 / 2, |u_h| the lowest
+     *     centre's wind speed and dz_1 the lowest layer's depth, times
+     *     exp(-((85000 - p) / 10000)^2) where p < 85000 Pa (no D0_col).
+     * The face diffusivity is the harmonic mean of the centre values.  The
+     * reverse mode and the column ensembles take closure 0 with H = INFINITY
+     * only (CA_ERR_ARG otherwise). */
+    int32_t vert_diff_model;
+    double vert_diff_H;     /* decay height [m] (0 or INFINITY: no decay)     */
+    double vert_diff_C_E;   /* bulk coefficient of VerticalDiffusion, >= 0    */
 } ca_config;
 
 /* Strided layout of one component: element (level v, node i, j, element h)

@@ -54,6 +54,7 @@ struct CaConfig
     alpha_w_col::Ptr{Float64}; vert_diff::Int32; D0_col::Ptr{Float64}; dt::Float64
     sfc_flux::Int32; Cd::Float64; Ch::Float64; gustiness::Float64; T_sfc_col::Ptr{Float64}
     implicit_diffusion::Int32; solve_iters::Int32
+    vert_diff_model::Int32; vert_diff_H::Float64; vert_diff_C_E::Float64
 end
 struct CaLayout
     offset::Int64; s_v::Int64; s_i::Int64; s_j::Int64; s_h::Int64

@@ -47,6 +47,7 @@ struct ora_ctx {
     real W[NN], D[NQ][NQ];
     /* precomputed */
     real *cu3, *K, *T, *p, *h, *fu3;
+    real *uh0;             /* horizontal wind speed of the lowest centre, per column */
     /* hyperdiffusion Laplacians */
     real *L1, *L2, *L3, *Ls;
     /* Jacobian bands (per column, level-major) */
@@ -139,6 +140,7 @@ int ora_create(const ora_geom *g, const ora_num *n, ora_ctx **out) {
     c->U = zalloc(sizeof(real) * c->S);
     c->Rv = zalloc(sizeof(real) * c->S);
     c->dY = zalloc(sizeof(real) * c->S);
+    c->uh0 = zalloc(sizeof(real) * c->ncol);
     *out = c;
     return 0;
 }
@@ -150,7 +152,7 @@ void ora_destroy(ora_ctx *c) {
                     c->Aru0, c->Aru1, c->Aeu0, c->Aeu1, c->Aur0, c->Aur1, c->Aue0, c->Aue1,
                     c->Auh0a, c->Auh0b, c->Auh1a, c->Auh1b, c->Auu_lo, c->Auu_d, c->Auu_up,
                     c->Aee_lo, c->Aee_d, c->Aee_up, c->Ahh_lo, c->Ahh_d, c->Ahh_up,
-                    c->Ut, c->U, c->Rv, c->dY};
+                    c->Ut, c->U, c->Rv, c->dY, c->uh0};
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); ++q) free(ptrs[q]);
     for (int q = 0; q < 12; ++q) free(c->t[q]);
     for (int q = 0; q < 4; ++q) {
@@ -420,6 +422,7 @@ static void set_precomputed(ora_ctx *c, real *Y) {
             real vv = (ua * ga * ua + ub * gb * ub) / 2;
             real Kk = (uh_dot + vv) / 2;
             c->K[q] = Kk;
+            if (k == 0) c->uh0[col] = (real)sqrt((double)uh_dot);
             real e_tot = rhoe / rho;
             real e_int = e_tot - Kk - c->Phic[k];
             real T = (real)c->n.T_0 + (e_int + (real)(c->n.R_d * c->n.T_0)) / (real)c->n.cv_d;
@@ -572,12 +575,38 @@ void ora_implicit_tendency(ora_ctx *c, ora_real *Yt, ora_real *Y) { implicit_ten
  * advdiv_matrix and grad_matrix are the centre divergence of a C3 face field
  * with zero boundary flux and the face gradient of a centre field, so row k
  * of D couples centres k - 1, k, k + 1 through faces k and k + 1. */
+/* Eddy diffusivity K_h of centre k (ᶜcompute_eddy_diffusivity_coefficient,
+ * cache/eddy_diffusivity_coefficient.jl:16-41):
+ *   DecayWithHeightDiffusion: D0 exp(-(z - z_sfc) / H)  (precomputed_quantities.jl:651-653),
+ *     D0 per column;
+ *   VerticalDiffusion: C_E |u_h| z_a with |u_h| the lowest centre's wind speed and
+ *     z_a half the lowest layer's depth, times exp(-((p_pbl - p) / p_strato)^2)
+ *     where p < p_pbl = 85000 Pa, p_strato = 10000 Pa (precomputed_quantities.jl:671-676).
+ * Reads the precomputed p and lowest-centre wind speed. */
+static real centre_K(const ora_ctx *c, int k, int col) {
+    if (c->n.vdiff_kind == 1) {
+        real za = (c->zf[1] - c->zf[0]) / 2;
+        real KE = (real)c->n.vdiff_C_E * c->uh0[col] * za;
+        real p = c->p[IDX(k, col)];
+        if (p > 85000) return KE;
+        real x = (85000 - p) / 10000;
+        return KE * (real)exp(-(double)(x * x));
+    }
+    return (real)c->n.D0_col[col] * (real)exp(-(double)(c->zc[k] - c->zf[0]) / c->n.vdiff_H);
+}
+
+/* rho K on face f = interp(rho) / interp(1 / max(K, eps))  (vertical_diffusion_boundary_layer.jl:86-90) */
+static real face_rhoK(const ora_ctx *c, const real *rho, int f, int col) {
+    real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
+    real Ka = centre_K(c, f - 1, col), Kb = centre_K(c, f, col);
+    Ka = Ka > eps ? Ka : eps;
+    Kb = Kb > eps ? Kb : eps;
+    return ((rho[IDX(f - 1, col)] + rho[IDX(f, col)]) / 2) / ((1 / Ka + 1 / Kb) / 2);
+}
+
 static void diffusion_blocks(ora_ctx *c, real *Y, real dtg, int col) {
     int nl = c->nl;
     real *rho = RHO(Y);
-    real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
-    real Kh = (real)c->n.D0_col[col];
-    real Km = Kh > eps ? Kh : eps;
     real cv = (real)c->n.cv_d, cp = (real)c->n.cp_d;
     for (int k = 0; k < nl; ++k) {
         int64_t q = IDX(k, col);
@@ -585,13 +614,13 @@ static void diffusion_blocks(ora_ctx *c, real *Y, real dtg, int col) {
         real dlo = 0, dup = 0;
         if (k >= 1) { /* face k */
             int f = k;
-            real rK = ((rho[IDX(f - 1, col)] + rho[q]) / 2) / ((1 / Km + 1 / Km) / 2);
+            real rK = face_rhoK(c, rho, f, col);
             real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
             dlo = c->Jf[IDX(f, col)] * g33 * rK / Jc;
         }
         if (k + 1 <= nl - 1) { /* face k + 1 */
             int f = k + 1;
-            real rK = ((rho[q] + rho[IDX(f, col)]) / 2) / ((1 / Km + 1 / Km) / 2);
+            real rK = face_rhoK(c, rho, f, col);
             real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);
             dup = c->Jf[IDX(f, col)] * g33 * rK / Jc;
         }
@@ -1050,7 +1079,7 @@ static void surface_flux_tendency(ora_ctx *c, real *Yt, real *Y) {
 
 /* ------------------------------------------------- vertical diffusion */
 /* vertical_diffusion_boundary_layer_tendency!, vertical_diffusion_boundary_layer.jl:63-103,
- * DecayWithHeightDiffusion(D0 = K_col, H = Inf): K_h = D0.  Added to Yt: by
+ * with K_h from centre_K (DecayWithHeightDiffusion or VerticalDiffusion).  Added to Yt: by
  * the explicit tendency (additional_tendency!, remaining_tendency.jl:185-190)
  * with diff_mode Explicit, by the implicit tendency (implicit_tendency.jl:
  * 73-82) with diff_mode Implicit.  Reads the precomputed T. */
@@ -1058,15 +1087,10 @@ static void vertical_diffusion_tendency(ora_ctx *c, real *Yt, real *Y) {
     int nl = c->nl, nc = c->ncol;
     int sphere = c->g.nelem > 0;
     real *rho = RHO(Y), *u1 = UH1(Y), *u2 = UH2(Y);
-    /* vertical_diffusion_boundary_layer_tendency!, vertical_diffusion_boundary_layer.jl:63-103,
-     * DecayWithHeightDiffusion(D0 = K_col, H = Inf): K_h = D0. */
     real cpd = (real)c->n.cp_d, T0 = (real)c->n.T_0;
-    real eps = (real)(sizeof(real) == 4 ? 1.1920929e-07 : 2.220446049250313e-16);
     const double *B = c->g.basis;
     ORA_PAR_FOR
     for (int col = 0; col < nc; ++col) {
-        real Kh = (real)c->n.D0_col[col];
-        real Km = Kh > eps ? Kh : eps;
         /* s_d flux: -rhoK grad_v(s_d); diffdiv with zero boundary flux */
         real *F = c->t[10], *Fu1 = c->t[0], *Fu2 = c->t[1], *Fu3 = c->t[2];
         for (int f = 0; f <= nl; ++f) {
@@ -1075,7 +1099,7 @@ static void vertical_diffusion_tendency(ora_ctx *c, real *Yt, real *Y) {
             Fu1[q] = Fu2[q] = Fu3[q] = 0;
             if (f == 0 || f == nl) continue;
             int64_t a = IDX(f - 1, col), b = IDX(f, col);
-            real rK = ((rho[a] + rho[b]) / 2) / ((1 / Km + 1 / Km) / 2);
+            real rK = face_rhoK(c, rho, f, col);
             real sda = cpd * (c->T[a] - T0) + c->Phic[f - 1];
             real sdb = cpd * (c->T[b] - T0) + c->Phic[f];
             real g33 = 1 / (c->Jvf[f] * c->Jvf[f]);

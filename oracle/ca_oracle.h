@@ -54,6 +54,9 @@ typedef struct ora_num {
     const double *T_sfc_col; /* per-column surface temperature [K]         */
     int32_t implicit_diffusion; /* vertical diffusion in T_imp (diff_mode Implicit) */
     int32_t solve_iters;   /* approximate_solve_iters of the linear solve    */
+    int32_t vdiff_kind;    /* 0 DecayWithHeightDiffusion, 1 VerticalDiffusion */
+    double vdiff_H;        /* decay height [m] (inf: K = D0)                 */
+    double vdiff_C_E;      /* VerticalDiffusion's bulk coefficient           */
 } ora_num;
 
 #ifdef ORACLE_F32

@@ -49,6 +49,7 @@ class _Num(C.Structure):
         ("sfc_flux", C.c_int32), ("Cd", C.c_double), ("Ch", C.c_double), ("gustiness", C.c_double),
         ("T_sfc_col", C.POINTER(C.c_double)),
         ("implicit_diffusion", C.c_int32), ("solve_iters", C.c_int32),
+        ("vdiff_kind", C.c_int32), ("vdiff_H", C.c_double), ("vdiff_C_E", C.c_double),
     ]
 
 
@@ -145,7 +146,10 @@ class Oracle:
         if alpha_w_col is not None:
             num.alpha_w_col = _dp(arr("aw", alpha_w_col))
         num.vert_diff = int(bool(n.vert_diff))
-        if n.vert_diff:
+        num.vdiff_kind = n.vert_diff_kind() if hasattr(n, "vert_diff_kind") else 0
+        num.vdiff_H = float(getattr(n, "vert_diff_H", float("inf")))
+        num.vdiff_C_E = float(getattr(n, "vert_diff_C_E", 0.044))
+        if n.vert_diff and num.vdiff_kind == 0:
             if D0_col is None:
                 raise ValueError("vertical diffusion needs D0_col")
             num.D0_col = _dp(arr("D0", D0_col))

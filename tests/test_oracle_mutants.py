@@ -116,9 +116,25 @@ MUTANTS = [
      'RHOE(Yt)[q] += (F[IDX(k + 1, col)] - F[IDX(k, col)]) / Jc;',
      'vdiff_column'),
     ('vertical diffusion: face rho from the lower centre',
-     'real rK = ((rho[a] + rho[b]) / 2) / ((1 / Km + 1 / Km) / 2);',
-     'real rK = rho[a] / ((1 / Km + 1 / Km) / 2);',
+     'return ((rho[IDX(f - 1, col)] + rho[IDX(f, col)]) / 2) / ((1 / Ka + 1 / Kb) / 2);',
+     'return rho[IDX(f - 1, col)] / ((1 / Ka + 1 / Kb) / 2);',
      'vdiff_sphere'),
+    ('vertical diffusion: arithmetic face mean of K',
+     '((1 / Ka + 1 / Kb) / 2)',
+     '(2 / (Ka + Kb))',
+     'vdiff_closures'),
+    ('DecayWithHeightDiffusion: decay height ignored',
+     '(real)exp(-(double)(c->zc[k] - c->zf[0]) / c->n.vdiff_H)',
+     '(real)1',
+     'vdiff_closures'),
+    ('VerticalDiffusion: taper sign',
+     'return KE * (real)exp(-(double)(x * x));',
+     'return KE * (real)exp((double)(x * x));',
+     'vdiff_closures'),
+    ('VerticalDiffusion: z_a the whole lowest layer',
+     'real za = (c->zf[1] - c->zf[0]) / 2;',
+     'real za = (c->zf[1] - c->zf[0]);',
+     'vdiff_closures'),
     ('vertical diffusion of u_h sign',
      'Fu1[q] = fac * (ub[0] - ua[0]) * inv;',
      'Fu1[q] = fac * (ua[0] - ub[0]) * inv;',

@@ -256,8 +256,60 @@ def check_vertical_diffusion_sphere(mk):
         assert min(_order(e[:, f])) > 1.9 and e[-1, f] < 3e-3, e
 
 
+def check_vertical_diffusion_closures(mk):
+    """The two closures of ᶜcompute_eddy_diffusivity_coefficient
+    (eddy_diffusivity_coefficient.jl:16-41), with the face diffusivity the
+    harmonic mean interp(rho) / interp(1 / K) (vertical_diffusion_boundary_layer.jl:86-90).
+    On uniform layers with constant rho and linear s_d the interior tendency is
+    rho s_d' (K_{k+1/2} - K_{k-1/2}) / dz exactly, which fixes the face K:
+    * DecayWithHeightDiffusion, K = D0 exp(-z / H): the harmonic face mean is
+      D0 exp(-z_f / H) / cosh(dz / 2H) (an arithmetic mean would multiply by
+      cosh instead);
+    * VerticalDiffusion, K = C_E |u_h(lowest centre)| dz_1 / 2, times
+      exp(-((85000 - p) / 10000)^2) where p < 85000 Pa: uniform below 850 hPa,
+      where the step equals DecayWithHeightDiffusion with D0 = C_E |u| dz / 2."""
+    N, Htop = 16, 4000.0
+    g = G.columns(1, G.uniform_faces(N, Htop))
+    zc, dz, zf = g.zc, Htop / N, g.z_face
+    off = P.Numerics(dt=10.0, hyperdiff=False)
+    rho0, Tb, Tt = 1.1, 300.0, 240.0
+    T = Tb + (Tt - Tb) * zc / Htop
+    sp = PAR.cp_d * (Tt - Tb) / Htop + PAR.grav  # d s_d / dz
+    Y = _column_state(zc, np.full(N, rho0), T, np.full(N, 5.0))
+
+    def vd(num, **kw):
+        return (_tendency(mk, g, num, Y, dss=False, **kw) - _tendency(mk, g, off, Y, dss=False))[3 * N:4 * N]
+
+    # DecayWithHeightDiffusion with H = 800 m
+    D0, H = 6.0, 800.0
+    dec = P.Numerics(dt=10.0, hyperdiff=False, vert_diff=True, vert_diff_H=H)
+    Kf = D0 * np.exp(-zf / H) / math.cosh(dz / (2 * H))
+    exp = rho0 * sp * (Kf[2:N] - Kf[1:N - 1]) / dz
+    np.testing.assert_allclose(vd(dec, D0_col=np.array([D0]))[1:-1], exp, rtol=1e-9)
+    # VerticalDiffusion: p = rho R_d T crosses 850 hPa inside the column
+    CE = 0.044
+    bulk = P.Numerics(dt=10.0, hyperdiff=False, vert_diff=True, vert_diff_model="VerticalDiffusion", vert_diff_C_E=CE)
+    p = rho0 * PAR.R_d * T
+    assert p[0] > 85000 > p[-1]
+    Kc = CE * 5.0 * dz / 2 * np.where(p > 85000, 1.0, np.exp(-((85000 - p) / 10000) ** 2))
+    Kf = 2 / (1 / Kc[:-1] + 1 / Kc[1:])  # interior faces 1 .. N - 1
+    exp = rho0 * sp * (Kf[1:] - Kf[:-1]) / dz
+    np.testing.assert_allclose(vd(bulk)[1:-1], exp, rtol=1e-9)
+    # below 850 hPa the closure is a uniform K = C_E |u| dz / 2
+    g2 = G.columns(1, G.uniform_faces(N, 800.0))
+    Y2 = _column_state(g2.zc, np.full(N, rho0), 300.0 - 0.006 * g2.zc, np.full(N, 5.0))
+    a = _tendency(mk, g2, bulk, Y2, dss=False)
+    b = _tendency(mk, g2, P.Numerics(dt=10.0, hyperdiff=False, vert_diff=True), Y2, dss=False,
+                  D0_col=np.array([CE * 5.0 * (800.0 / N) / 2]))
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
 def test_vertical_diffusion_column_closed_forms():
     check_vertical_diffusion_column(default_factory)
+
+
+def test_vertical_diffusion_closures():
+    check_vertical_diffusion_closures(default_factory)
 
 
 def test_vertical_diffusion_deep_sphere_second_order():
@@ -557,6 +609,7 @@ ALL_CHECKS = {
     "hyperdiffusion": check_hyperdiffusion,
     "vdiff_column": check_vertical_diffusion_column,
     "vdiff_sphere": check_vertical_diffusion_sphere,
+    "vdiff_closures": check_vertical_diffusion_closures,
     "rayleigh_uh": check_rayleigh_uh,
     "sfc_column": check_surface_flux_column,
     "sfc_sphere": check_surface_flux_sphere,

@@ -260,6 +260,46 @@ def copy_calibration(torch):
     torch.cuda.synchronize()
 
 
+def step_graph_sizes(torch, par, steps=50):
+    """ms per step of ca_step against its hipGraph replay (ca_step_graph) at
+    the sizes of the workloads: h8 z30 (the differentiation workload's grid),
+    h16 z63 (the 50-step sphere case) and h30 z63.  One call of `steps`
+    steps each, after a warm-up call (which also captures the graph)."""
+    from cadycore import geometry as G
+    from cadycore import initial as I
+    from cadycore import params as P
+    from cadycore.lib import DryDycore
+
+    out = {}
+    side = torch.cuda.Stream()
+    for he, ze, dzb, dt in ((8, 30, 500.0, 150.0), (16, 63, 30.0, 100.0), (30, 63, 30.0, 60.0)):
+        g = G.cubed_sphere(he, G.tanh_stretched_faces(ze, 30000.0, dzb), par.planet_radius, par.Omega)
+        d = DryDycore(g, par, P.Numerics(dt=dt))
+        Y0 = d.state(I.sphere_state(g, par, seed=2))
+        Y = Y0.clone()
+        res = {}
+        for name in ("eager", "graph"):
+            def run(n):
+                if name == "eager":
+                    d.step(Y, dt, n)
+                    torch.cuda.synchronize()
+                else:
+                    d.step_graph(Y, dt, n, side)
+                    side.synchronize()
+            Y.copy_(Y0)
+            torch.cuda.synchronize()
+            run(3)
+            Y.copy_(Y0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run(steps)
+            res[name] = (time.perf_counter() - t0) / steps * 1e3
+        out[f"h{he}_z{ze}"] = {"eager_ms_per_step": round(res["eager"], 4), "graph_ms_per_step": round(res["graph"], 4),
+                              "speedup": round(res["eager"] / res["graph"], 3)}
+        d.close()
+    return out
+
+
 def step_variants(torch, grid, par, Y0, dt, steps=30, warmup=5):
     """ms per step of the same workload with the optional terms on: the surface
     fluxes (ExchangeCoefficients Cd = Ch = 0.0044, the analytic surface
@@ -316,7 +356,8 @@ def main():
         zf = G.tanh_stretched_faces(a.z_elem, 30000.0, 30.0)
         grid = G.cubed_sphere(a.h_elem, zf, par.planet_radius, par.Omega)
         out = step_variants(torch, grid, par, I.sphere_state(grid, par, seed=2), a.dt, steps=a.variant_steps)
-        print(json.dumps({"workload": f"h_elem={a.h_elem} z_elem={a.z_elem} dt={a.dt:g}s", "step_variants": out}))
+        print(json.dumps({"workload": f"h_elem={a.h_elem} z_elem={a.z_elem} dt={a.dt:g}s", "step_variants": out,
+                          "step_graph": step_graph_sizes(torch, par)}))
         return 0
     if a.adjoint_only or a.ensemble_only:
         out = adjoint_c4(torch, par) if a.adjoint_only else ensemble_c3(torch, par)
@@ -380,6 +421,7 @@ def main():
     if a.copy_calibration:
         copy_calibration(torch)
     variants = None if a.no_variants else step_variants(torch, grid, par, Y0, a.dt, steps=a.variant_steps)
+    graphs = None if a.no_variants else step_graph_sizes(torch, par)
     ens = None if a.no_ensemble else ensemble_c3(torch, par)
     adj = None if a.no_adjoint else adjoint_c4(torch, par)
 
@@ -415,6 +457,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_variants": variants,
+        "step_graph": graphs,
         "c3_ensemble": ens,
         "c4_adjoint": adj,
         "kernels": kernels,

@@ -77,7 +77,7 @@ class CaFieldVector(C.Structure):
 EXPORTED = (
     "ca_create", "ca_destroy", "ca_state_size", "ca_last_error", "ca_set_precomputed",
     "ca_remaining_tendency", "ca_implicit_tendency", "ca_update_jacobian", "ca_ldiv",
-    "ca_correct_implicit_advection", "ca_dss", "ca_step", "ca_pack_state", "ca_unpack_state",
+    "ca_correct_implicit_advection", "ca_dss", "ca_step", "ca_step_graph", "ca_pack_state", "ca_unpack_state",
     "ca_check_finite", "ca_profile_enable", "ca_profile_count", "ca_profile_get",
     "ca_set_tuning", "ca_get_tuning",
     "ca_ensemble_create", "ca_ensemble_destroy", "ca_ensemble_state_size", "ca_ensemble_member_offset",
@@ -113,6 +113,7 @@ def load_library(path: str = LIB_PATH):
         getattr(lib, nm).argtypes = [vp, vp, vp, vp]
     lib.ca_update_jacobian.argtypes = [vp, vp, C.c_double, vp]
     lib.ca_step.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
+    lib.ca_step_graph.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
     fvp = C.POINTER(CaFieldVector)
     lib.ca_fieldvector_native.argtypes = [vp, vp, fvp]
     lib.ca_fieldvector_vijfh.argtypes = [vp, vp, vp, fvp]
@@ -316,6 +317,12 @@ class DryDycore:
 
     def step(self, Y, dt, nsteps=1):
         self._check(self.lib.ca_step(self.h, self._ptr(Y), float(dt), int(nsteps), self._stream()), "step!")
+
+    def step_graph(self, Y, dt, nsteps, stream):
+        """``step`` replayed from a captured hipGraph on ``stream`` (a
+        torch.cuda.Stream other than the default one)."""
+        self._check(self.lib.ca_step_graph(self.h, self._ptr(Y), float(dt), int(nsteps),
+                                           C.c_void_p(stream.cuda_stream)), "step (graph)")
 
     # ------------------------------------- hooks on ClimaCore FieldVectors
     def fieldvector(self, Y=None, c=None, f=None):

@@ -62,6 +62,11 @@ struct ca_handle {
     double aE[4][4], aI[4][4], b[4];
     int newton_variant = 1;
     bool vd_plain = true;  // no vertical diffusion, or DecayWithHeightDiffusion with H = inf (the reverse mode's)
+    // ca_step_graph: one step's launches captured for replay, with what they bake in
+    hipGraphExec_t gexec = nullptr;
+    float *g_Y = nullptr;
+    double g_dt = 0.0;
+    hipStream_t g_st = nullptr;
     int explicit_chunk = 8;
     int dss_chunk = 8;
     int lap_chunk = 4;  // levels per work item of k_laplacians
@@ -935,8 +940,10 @@ ca_status ca_create(const ca_geometry *geom, const ca_params *par, const ca_conf
     return CA_OK;
 }
 
+static void drop_step_graph(ca_handle *h);
 ca_status ca_destroy(ca_handle *h) {
     if (!h) return CA_ERR_ARG;
+    drop_step_graph(h);
     for (auto &r : h->recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -1041,14 +1048,11 @@ ca_status ca_dss(ca_handle *h, float *Y, void *stream) {
     return dss_state(h, Y, (hipStream_t)stream, KID_DSS_STATE);
 }
 
-ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream) {
-    if (!h || !Y || nsteps < 0 || !(dt > 0)) return CA_ERR_ARG;
-    hipStream_t st = (hipStream_t)stream;
+// the step starts from a continuous, filtered state (the reference's state
+// after dss! and cache! at the end of every step)
+static ca_status step_prologue(ca_handle *h, float *Y, hipStream_t st) {
     const DevGeom &g = h->g;
     ca_status s;
-    if ((s = poison_work(h, st))) return s;
-    // the step starts from a continuous, filtered state (the reference's
-    // state after dss! and cache! at the end of every step)
     if (g.nelem > 0) {
         HIPCHK(h, hipMemcpyAsync(h->U, Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
         if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
@@ -1060,8 +1064,59 @@ ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *strea
         if ((s = check_launch(h, "k_combine"))) return s;
         if ((s = filter_columns(h, h->U, Y, st))) return s;
     }
+    return CA_OK;
+}
+
+static void drop_step_graph(ca_handle *h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+    h->g_Y = nullptr;
+}
+
+ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream) {
+    if (!h || !Y || nsteps < 0 || !(dt > 0)) return CA_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    ca_status s;
+    if ((s = poison_work(h, st))) return s;
+    if ((s = step_prologue(h, Y, st))) return s;
     for (int stp = 0; stp < nsteps; ++stp)
         if ((s = one_step(h, Y, dt, st, nullptr))) return s;
+    return CA_OK;
+}
+
+ca_status ca_step_graph(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream) {
+    if (!h || !Y || nsteps < 0 || !(dt > 0)) return CA_ERR_ARG;
+    if (!stream) return fail(h, CA_ERR_ARG, "ca_step_graph: needs a non-default stream (graph capture)");
+    if (h->prof || h->poison)
+        return fail(h, CA_ERR_ARG, "ca_step_graph: turn profiling and debug poisoning off (per-launch host work)");
+    hipStream_t st = (hipStream_t)stream;
+    ca_status s;
+    if ((s = step_prologue(h, Y, st))) return s;
+    if (nsteps == 0) return CA_OK;
+    if (!h->gexec || h->g_Y != Y || h->g_dt != dt || h->g_st != st) {
+        drop_step_graph(h);
+        // buffers a stage allocates on first use are allocated before the capture
+        if (h->n.idiff && (s = ensure_bands(h))) return s;
+        HIPCHK(h, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        ca_status cs = one_step(h, Y, dt, st, nullptr);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (cs) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return cs;
+        }
+        if (e != hipSuccess) return fail(h, CA_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            h->gexec = nullptr;
+            return fail(h, CA_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        h->g_Y = Y;
+        h->g_dt = dt;
+        h->g_st = st;
+    }
+    for (int stp = 0; stp < nsteps; ++stp) HIPCHK(h, hipGraphLaunch(h->gexec, st));
     return CA_OK;
 }
 
@@ -1154,6 +1209,7 @@ ca_status ca_set_debug_poison(ca_handle *h, int32_t on) {
 
 ca_status ca_set_tuning(ca_handle *h, int32_t knob, int32_t value) {
     if (!h) return CA_ERR_ARG;
+    drop_step_graph(h);  // a captured step bakes in the launch schedules
     if (knob == CA_TUNE_NEWTON) {
         if (value < 0 || value > 2) return fail(h, CA_ERR_ARG, "CA_TUNE_NEWTON must be 0, 1 or 2");
         if (value > 0 && h->g.nlev + 1 > 64)

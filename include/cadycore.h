@@ -159,6 +159,15 @@ ca_status ca_dss(ca_handle *h, float *Y, void *stream);
 
 /* fused ARS343 step(s): Y <- step^nsteps(Y) */
 ca_status ca_step(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream);
+/* ca_step as a hipGraph replay.  The first call captures one step's launches
+ * on `stream` into a graph the handle keeps; later calls with the same Y,
+ * dt and stream replay it nsteps times, and any other Y, dt or stream (or a
+ * ca_set_tuning in between) captures again.  Bitwise the results of ca_step.
+ * `stream` must be a non-default stream; profiling and debug poisoning must
+ * be off (CA_ERR_ARG otherwise).  On small grids, where a step is many short
+ * launches, the replay removes their per-launch host cost.  No counterpart
+ * in the reference (CTS.step! launches through Julia). */
+ca_status ca_step_graph(ca_handle *h, float *Y, double dt, int32_t nsteps, void *stream);
 
 /* layout conversion: 5 components (rho, u_1, u_2, rho e_tot on nlev levels;
  * u_3 on nlev + 1 levels) between strided caller arrays and native Y */

@@ -234,9 +234,10 @@ ca_status reverse_step(ca_handle *h, float *rec, double dt, hipStream_t st) {
                           {Utb[3], ae(3), ai(3)}},
                          false, PL_ALL, st, fib, PL_FI)))
             return s;
-        // F_E_i = DSS(T_E(U_i))
-        if ((s = dss_T_state(h, feb, gb, 0, st))) return s;
-        if ((s = explicit_vjp(h, rec_U(h, rec, i), rec_L(h, rec, i), gb, Ubs, st, false))) return s;
+        // F_E_i = DSS(T_E(U_i)) for i < 3; F_E_3 = T_E(U_3), which only the
+        // end-of-step DSS averages (one_step)
+        if (i < 3 && (s = dss_T_state(h, feb, gb, 0, st))) return s;
+        if ((s = explicit_vjp(h, rec_U(h, rec, i), rec_L(h, rec, i), i < 3 ? gb : feb, Ubs, st, false))) return s;
         // U_i = Newton(U~_i); F_I_i = (U_i - U~_i) / dtg and the Laplacian
         // face terms are added to the U_i cotangent as the kernel loads it
         if ((s = newton_vjp(h, rec_Ut(h, rec, i), rec_U(h, rec, i), Ubs, (float)dtg, Utb[i], fib, (float)(-1.0 / dtg),

@@ -27,11 +27,11 @@ sys.path.insert(0, os.path.join(ROOT, "climaatmos.jl_amd"))
 METRIC = "SYPD + achieved HBM GB/s, dry global h30 z63 Nq4 FP32; adjoint/forward ratio"
 HBM_PEAK_GBPS = 8000.0
 # launches per ARS343 step of each kernel of kernel_bytes (the step's
-# compulsory traffic is their weighted sum: 55.2 state sizes at h30 z63).
-# The tendencies of stages 1-3 are DSSed; the last stage's is averaged by the
-# end-of-step DSS of the state.
-LAUNCHES_PER_STEP = {"k_laplacians": 4, "k_dss(laplacians)": 4, "k_explicit": 4, "k_dss(tendency)": 3,
-                     "k_newton_stage": 3, "k_final": 1, "k_dss(state)": 1}
+# compulsory traffic is their weighted sum: 51.4 state sizes at h30 z63).
+# The tendencies of stages 1-3 are DSSed; the last explicit launch writes the
+# step's final combination, which the end-of-step DSS of the state averages.
+LAUNCHES_PER_STEP = {"k_laplacians": 4, "k_dss(laplacians)": 4, "k_explicit": 3, "k_explicit_final": 1,
+                     "k_dss(tendency)": 3, "k_newton_stage": 3, "k_dss(state)": 1}
 
 
 def parse():
@@ -68,19 +68,20 @@ def kernel_bytes(S, nC, nF):
     """Algorithmic (compulsory) bytes per launch of each kernel of the step,
     keyed by the profile names of the library.  fi = one implicit tendency
     (rho, rho e_tot and u_3 planes).  Newton stage i (i = 1, 2, 3) reads u,
-    i explicit and i - 1 implicit tendencies and writes U and F_I; the three
-    stages are averaged, as their launches are.  The final combination reads
-    u, the explicit tendencies of stages 1..3 (ARS343's b_0 is 0, so F_E_0 is
-    not read) and 3 implicit tendencies and writes u."""
+    i explicit and i - 1 implicit tendencies and writes U and (i < 3) F_I;
+    the three stages are averaged, as their launches are.  The last explicit launch
+    (k_explicit_final) also forms the final combination U_4 + dt sum_j c_j
+    F_E_j: it reads the three earlier explicit tendencies besides its usual
+    inputs and writes u_{n+1} in place of a tendency."""
     fi = 2 * nC + nF
-    newton = sum((S + i * S + (i - 1) * fi) + (S + fi) for i in (1, 2, 3)) / 3.0
+    newton = sum((S + i * S + (i - 1) * fi) + (S + (fi if i < 3 else 0)) for i in (1, 2, 3)) / 3.0
     return {
         "k_laplacians": 4 * (S + 4 * nC),
         "k_dss(laplacians)": 4 * (2 * 4 * nC),
         "k_explicit": 4 * (S + 4 * nC + S),
+        "k_explicit_final": 4 * (S + 4 * nC + 3 * S + S),
         "k_dss(tendency)": 4 * (2 * S),
         "k_newton_stage": 4 * newton,
-        "k_final": 4 * (S + 3 * S + 3 * fi + S),
         "k_dss(state)": 4 * (2 * S),
     }
 

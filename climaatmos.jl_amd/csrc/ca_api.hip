@@ -28,10 +28,12 @@ enum KernelId {
     KID_FINAL,
     KID_DSS_STATE,
     KID_HOOKS,
+    KID_EXPLICIT_FINAL,  // the last explicit tendency with the final combination
     KID_COUNT
 };
-const char *kKernelNames[KID_COUNT] = {"k_laplacians", "k_explicit", "k_dss(tendency)", "k_dss(laplacians)",
-                                       "k_newton_stage", "k_final", "k_dss(state)", "hook kernels"};
+const char *kKernelNames[KID_COUNT] = {"k_laplacians",   "k_explicit",   "k_dss(tendency)", "k_dss(laplacians)",
+                                       "k_newton_stage", "k_final",      "k_dss(state)",    "hook kernels",
+                                       "k_explicit_final"};
 
 struct ProfRec {
     int id;
@@ -299,12 +301,34 @@ ca_status dss_state(ca_handle *h, float *Y, hipStream_t st, int kid) {
     return dss_gather_state(h, h->raw, Y, 0, st, kid);
 }
 
+template <bool FIN>
+void launch_explicit(ca_handle *h, const float *Y, const float *Lw, float *out, const FinFuse &fin, hipStream_t st) {
+    const DevGeom &g = h->g;
+    const DevNum &n = h->n;
+    const int ch = h->explicit_chunk;
+    const int nchunks = ch > 0 ? (int)cdiv(g.nlev, ch) : 1;
+    const dim3 grid(cdiv(g.ncol, 256), nchunks), blk(256);
+    // with implicit diffusion the vertical diffusion is in the implicit tendency
+    const bool hy = n.hyper != 0, vd = n.vdiff != 0 && !n.idiff;
+    if (n.sphere) {
+        if (hy && !vd) k_explicit<true, true, false, FIN><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch, fin);
+        else if (!hy && !vd) k_explicit<true, false, false, FIN><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch, fin);
+        else if (hy && vd) k_explicit<true, true, true, FIN><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch, fin);
+        else k_explicit<true, false, true, FIN><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch, fin);
+    } else {
+        if (vd) k_explicit<false, false, true, FIN><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch, fin);
+        else k_explicit<false, false, false, FIN><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch, fin);
+    }
+}
+
 // remaining_tendency!: Laplacians, their DSS, the fused explicit kernel.
 // With dss_out, the tendency is written un-DSSed to the scratch buffer and
 // gathered (DSSed) into Yt; otherwise it goes to Yt as the reference's
-// remaining_tendency! returns it.
+// remaining_tendency! returns it.  With fin, Yt receives the step's final
+// combination Y + fin's terms instead (un-DSSed; Yt must not be h->raw's
+// Laplacian use, which ends before the explicit kernel starts).
 ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t st, bool dss_out = false,
-                            float *Lout = nullptr) {
+                            float *Lout = nullptr, const FinFuse *fin = nullptr) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
     const bool dss = n.sphere;
@@ -320,26 +344,17 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
             launch_dss<4, 0, false>(h, h->raw, Lw, 0, st);
         }
     }
+    if (fin) dss_out = false;
     float *out = (dss_out && dss) ? h->raw : Yt;
     {
-        Launch l(h, st, KID_EXPLICIT);
-        const int ch = h->explicit_chunk;
-        const int nchunks = ch > 0 ? (int)cdiv(g.nlev, ch) : 1;
-        const dim3 grid(cdiv(g.ncol, 256), nchunks), blk(256);
-        // with implicit diffusion the vertical diffusion is in the implicit tendency
-        const bool hy = n.hyper != 0, vd = n.vdiff != 0 && !n.idiff;
-        if (n.sphere) {
-            if (hy && !vd) k_explicit<true, true, false><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
-            else if (!hy && !vd) k_explicit<true, false, false><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
-            else if (hy && vd) k_explicit<true, true, true><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
-            else k_explicit<true, false, true><<<grid, blk, 0, st>>>(g, n, Y, Lw, out, ch);
-        } else {
-            if (vd) k_explicit<false, false, true><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
-            else k_explicit<false, false, false><<<grid, blk, 0, st>>>(g, n, Y, h->L, out, ch);
-        }
+        Launch l(h, st, fin ? KID_EXPLICIT_FINAL : KID_EXPLICIT);
+        if (fin)
+            launch_explicit<true>(h, Y, Lw, out, *fin, st);
+        else
+            launch_explicit<false>(h, Y, Lw, out, FinFuse{}, st);
         // surface fluxes at the first centre (surface_flux_tendency!, called
         // from additional_tendency!: part of the tendency the DSS averages)
-        if (n.sflux) k_sfc_flux<<<cdiv(g.ncol, 256), 256, 0, st>>>(g, n, Y, out);
+        if (n.sflux) k_sfc_flux<<<cdiv(g.ncol, 256), 256, 0, st>>>(g, n, Y, out, fin ? fin->cg : 1.0f);
     }
     ca_status s = check_launch(h, "explicit tendency");
     if (s) return s;
@@ -350,41 +365,45 @@ ca_status explicit_tendency(ca_handle *h, float *Yt, const float *Y, hipStream_t
 inline bool ca_al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 template <int TC, bool V4>
-void launch_pcr_tcv(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st) {
+void launch_pcr_tcv(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st,
+                    bool keep_fi) {
+    float *FI = keep_fi ? h->FI[i] : nullptr;
     const DevGeom &g = h->g;
     const dim3 grid(cdiv(g.ncol, TC)), blk(256);
     if (src.nfull == 2 && src.nfi == 0)
-        k_newton_pcr<TC, 2, 0, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, 2, 0, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, FI, Ut);
     else if (src.nfull == 3 && src.nfi == 1)
-        k_newton_pcr<TC, 3, 1, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, 3, 1, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, FI, Ut);
     else if (src.nfull == 4 && src.nfi == 2)
-        k_newton_pcr<TC, 4, 2, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, 4, 2, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, FI, Ut);
     else
-        k_newton_pcr<TC, -1, -1, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, h->FI[i], Ut);
+        k_newton_pcr<TC, -1, -1, V4><<<grid, blk, 0, st>>>(g, h->n, src, dtg, U, FI, Ut);
 }
 
 // 16-byte row accesses when every row of every plane is 16-byte aligned:
 // ncol a multiple of 4 and all base pointers 16-byte aligned
 template <int TC>
-void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st) {
+void launch_pcr_tc(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, float *Ut, hipStream_t st,
+                   bool keep_fi) {
     bool v4 = TC == 16 && h->g.ncol % 4 == 0 && ca_al16(U) && ca_al16(h->FI[i]) && (!Ut || ca_al16(Ut));
     for (int t = 0; t < src.nfull; ++t) v4 = v4 && ca_al16(src.full[t]);
     for (int t = 0; t < src.nfi; ++t) v4 = v4 && ca_al16(src.fi[t]);
     if (v4)
-        launch_pcr_tcv<16, true>(h, src, dtg, i, U, Ut, st);
+        launch_pcr_tcv<16, true>(h, src, dtg, i, U, Ut, st, keep_fi);
     else
-        launch_pcr_tcv<TC, false>(h, src, dtg, i, U, Ut, st);
+        launch_pcr_tcv<TC, false>(h, src, dtg, i, U, Ut, st, keep_fi);
 }
 
 // the implicit stage i from src into U (h->U unless the caller records);
-// Ut (optional) receives the assembled U~
+// Ut (optional) receives the assembled U~; keep_fi = false leaves F_I
+// unstored (the step's last stage: the final combination does not read it)
 void launch_newton_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, hipStream_t st, float *U = nullptr,
-                       float *Ut = nullptr) {
+                       float *Ut = nullptr, bool keep_fi = true) {
     if (!U) U = h->U;
     if (h->newton_variant == 2)
-        launch_pcr_tc<32>(h, src, dtg, i, U, Ut, st);
+        launch_pcr_tc<32>(h, src, dtg, i, U, Ut, st, keep_fi);
     else
-        launch_pcr_tc<16>(h, src, dtg, i, U, Ut, st);
+        launch_pcr_tc<16>(h, src, dtg, i, U, Ut, st, keep_fi);
 }
 
 ca_status ensure_bands(ca_handle *h) {
@@ -781,10 +800,10 @@ std::string check_create_args(const ca_geometry *geom, const ca_params *par, con
 // at U~, dU = solve(A, R) into U, U = U~ - dU, the post-Newton upwind
 // correction at U, and F_I = (U - U~) / dtg with all five components.
 template <bool V4>
-void launch_idiff_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st) {
+void launch_idiff_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st, bool keep_fi) {
     const DevGeom &g = h->g;
     const unsigned nb = cdiv(g.ncol, 16);
-    float *FI = h->FI[i];
+    float *FI = keep_fi ? h->FI[i] : nullptr;
     if (src.nfull == 2 && src.nfi == 0)
         k_newton_idiff_pcr<2, 0, V4><<<nb, 256, 0, st>>>(g, h->n, src, dtg, U, FI);
     else if (src.nfull == 3 && src.nfi == 1)
@@ -795,7 +814,10 @@ void launch_idiff_pcr(ca_handle *h, const StageSrc &src, float dtg, int i, float
         k_newton_idiff_pcr<-1, -1, V4><<<nb, 256, 0, st>>>(g, h->n, src, dtg, U, FI);
 }
 
-ca_status idiff_stage(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st) {
+// keep_fi = false: the fused stage does not store F_I (the step's last stage,
+// whose F_I nothing reads); the hook-kernel stage uses F_I as its scratch.
+ca_status idiff_stage(ca_handle *h, const StageSrc &src, float dtg, int i, float *U, hipStream_t st,
+                      bool keep_fi = true) {
     const DevGeom &g = h->g;
     const DevNum &n = h->n;
     ca_status s;
@@ -804,9 +826,9 @@ ca_status idiff_stage(ca_handle *h, const StageSrc &src, float dtg, int i, float
         for (int t = 0; t < src.nfull; ++t) v4 = v4 && ca_al16(src.full[t]);
         for (int t = 0; t < src.nfi; ++t) v4 = v4 && ca_al16(src.fi[t]);
         if (v4)
-            launch_idiff_pcr<true>(h, src, dtg, i, U, st);
+            launch_idiff_pcr<true>(h, src, dtg, i, U, st, keep_fi);
         else
-            launch_idiff_pcr<false>(h, src, dtg, i, U, st);
+            launch_idiff_pcr<false>(h, src, dtg, i, U, st, keep_fi);
         return check_launch(h, "implicit-diffusion stage");
     }
     if ((s = ensure_bands(h))) return s;
@@ -849,6 +871,17 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
     // P is linear and P(P x) = P x, so P(c + b_4 dt F_E,4) equals
     // P(c + b_4 dt P F_E,4) up to rounding, and one DSS pass is saved.
     const int last = 3;
+    // The final combination is fused into the last explicit tendency, which
+    // writes u_{n+1} (before its DSS) to h->raw: ARS343's implicit tableau is
+    // stiffly accurate (a_I[3][j] = b[j] for every j, set together in
+    // ars343), so u + dt sum_j b_j (F_E,j + F_I,j) = U_4 + dt sum_j
+    // (b_j - a_E[3][j]) F_E,j.
+    FinFuse fin{};
+    for (int j = 0; j < 3; ++j) {
+        fin.fe[j] = h->FE[j];
+        fin.c[j] = (float)(dt * (h->b[j] - h->aE[3][j]));
+    }
+    fin.cg = (float)(dt * h->b[3]);
     if (rec) HIPCHK(h, hipMemcpyAsync(rec_U(h, rec, 0), Y, (size_t)h->S * sizeof(float), hipMemcpyDeviceToDevice, st));
     if ((s = explicit_tendency(h, h->FE[0], Y, st, true, rec ? rec_L(h, rec, 0) : nullptr))) return s;
     for (int i = 1; i < 4; ++i) {
@@ -873,9 +906,13 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
         if (h->n.idiff) {  // the implicit stage from the hook kernels (no records: no reverse mode)
             {
                 Launch l(h, st, KID_NEWTON_STAGE);
-                if ((s = idiff_stage(h, src, dtg, i, h->U, st))) return s;
+                if ((s = idiff_stage(h, src, dtg, i, h->U, st, i < last))) return s;
             }
-            if ((s = explicit_tendency(h, h->FE[i], h->U, st, i < last, nullptr))) return s;
+            if (i < last) {
+                if ((s = explicit_tendency(h, h->FE[i], h->U, st, true, nullptr))) return s;
+            } else if ((s = explicit_tendency(h, h->raw, h->U, st, false, nullptr, &fin))) {
+                return s;
+            }
             continue;
         }
         const bool pcr = h->newton_variant == 1 || h->newton_variant == 2;
@@ -887,37 +924,20 @@ ca_status one_step(ca_handle *h, float *Y, double dt, hipStream_t st, float *rec
         {
             Launch l(h, st, KID_NEWTON_STAGE);
             if (pcr)
-                launch_newton_pcr(h, src, dtg, i, st, Ui, rec ? rec_Ut(h, rec, i) : nullptr);
+                launch_newton_pcr(h, src, dtg, i, st, Ui, rec ? rec_Ut(h, rec, i) : nullptr, i < last);
             else
                 k_newton_stage<<<cdiv(g.ncol, 128), 128, 0, st>>>(g, h->n, src, dtg, Ui, h->FI[i], h->scr);
         }
         if ((s = check_launch(h, "k_newton_stage"))) return s;
-        if ((s = explicit_tendency(h, h->FE[i], Ui, st, i < last, rec ? rec_L(h, rec, i) : nullptr))) return s;
+        if ((s = explicit_tendency(h, i < last ? h->FE[i] : h->raw, Ui, st, i < last,
+                                   rec ? rec_L(h, rec, i) : nullptr, i < last ? nullptr : &fin)))
+            return s;
     }
-    // u_{n+1} = u + dt sum b (F_E + F_I), then dss! and the filter
-    FinalSrc fs{};
-    fs.dt = (float)dt;
-    fs.fi_full = h->n.idiff;
-    for (int j = 0; j < 4; ++j)
-        if (h->b[j] != 0.0) {
-            fs.fe[fs.nfe] = h->FE[j];
-            fs.be[fs.nfe] = (float)h->b[j];
-            ++fs.nfe;
-            if (j >= 1) {
-                fs.fi[fs.nfi] = h->FI[j];
-                fs.bi[fs.nfi] = (float)h->b[j];
-                ++fs.nfi;
-            }
-        }
-    {
-        Launch l(h, st, KID_FINAL);
-        k_combine<<<cdiv(cdiv(h->S, 4), 256), 256, 0, st>>>(g, Y, h->U, fs);
-    }
-    if ((s = check_launch(h, "k_combine"))) return s;
+    // u_{n+1} (in h->raw), then dss! and the filter
     if (g.nelem > 0) {
-        if ((s = dss_gather_state(h, h->U, Y, 1, st, KID_DSS_STATE))) return s;
+        if ((s = dss_gather_state(h, h->raw, Y, 1, st, KID_DSS_STATE))) return s;
     } else {
-        if ((s = filter_columns(h, h->U, Y, st))) return s;
+        if ((s = filter_columns(h, h->raw, Y, st))) return s;
     }
     return CA_OK;
 }

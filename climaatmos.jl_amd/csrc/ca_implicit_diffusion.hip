@@ -688,12 +688,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
 #pragma unroll
                 for (int comp = 0; comp < 4; ++comp) {
                     *reinterpret_cast<float4 *>(U + comp * nC + q) = row4(tU[comp]);
-                    *reinterpret_cast<float4 *>(FI + comp * nC + q) = row4(tF[comp]);
+                    if (FI) *reinterpret_cast<float4 *>(FI + comp * nC + q) = row4(tF[comp]);
                 }
             }
             if (r <= nl) {
                 *reinterpret_cast<float4 *>(U + 4 * nC + q) = row4(tU[4]);
-                *reinterpret_cast<float4 *>(FI + 4 * nC + q) = row4(tF[4]);
+                if (FI) *reinterpret_cast<float4 *>(FI + 4 * nC + q) = row4(tF[4]);
             }
         }
         return;
@@ -707,12 +707,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
 #pragma unroll
             for (int comp = 0; comp < 4; ++comp) {
                 U[comp * nC + q] = tU[comp][r][cc];
-                FI[comp * nC + q] = tF[comp][r][cc];
+                if (FI) FI[comp * nC + q] = tF[comp][r][cc];
             }
         }
         if (r <= nl) {
             U[4 * nC + q] = tU[4][r][cc];
-            FI[4 * nC + q] = tF[4][r][cc];
+            if (FI) FI[4 * nC + q] = tF[4][r][cc];
         }
     }
 }

@@ -59,10 +59,24 @@ __global__ void k_newton_idiff_pcr(DevGeom g, DevNum n, StageSrc src, float dtg,
 template <int TC, int NF, int NI, bool V4>
 __global__ void k_newton_pcr(DevGeom g, DevNum n, StageSrc src, float dtg, float *U, float *FI, float *Ut);
 
+// The final combination of an ARS343 step fused into its last explicit
+// tendency.  The implicit tableau is stiffly accurate (b_j = a_I[4][j]) and
+// F_I,j = (U_j - U~_j) / (dt a_I[j][j]), so
+//   u + dt sum_j b_j (F_E,j + F_I,j) = U_4 + dt sum_j (b_j - a_E[4][j]) F_E,j
+// with a_E[4][4] = 0: the last stage's tendency need not be stored, nor the
+// implicit ones read.
+struct FinFuse {
+    const float *fe[3];  // F_E of stages 1..3
+    float c[3];          // dt (b_j - a_E[4][j])
+    float cg;            // dt b_4, the weight of the tendency being computed
+};
+
 // explicit tendency (ca_spectral.hip)
 __global__ void k_laplacians(DevGeom g, DevNum n, const float *Y, float *L, int chunk);
-template <bool SPH, bool HYPER, bool VDIFF>
-__global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk);
+// FIN: Yt receives U + the combination of FinFuse instead of the tendency at U
+template <bool SPH, bool HYPER, bool VDIFF, bool FIN>
+__global__ void k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk,
+                           FinFuse fin);
 
 // DSS (ca_dss.hip): out-of-place weighted average over shared nodes.
 // NC centre planes, the vector pair at planes (VEC, VEC + 1) (-1: none), a

@@ -308,12 +308,14 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
             if (r < nl) {
 #pragma unroll
                 for (int comp = 0; comp < 4; ++comp) *reinterpret_cast<float4 *>(U + comp * nC + q) = row4(tU[comp]);
-                *reinterpret_cast<float4 *>(FI + q) = row4(tF[0]);
-                *reinterpret_cast<float4 *>(FI + nC + q) = row4(tF[1]);
+                if (FI) {
+                    *reinterpret_cast<float4 *>(FI + q) = row4(tF[0]);
+                    *reinterpret_cast<float4 *>(FI + nC + q) = row4(tF[1]);
+                }
             }
             if (r <= nl) {
                 *reinterpret_cast<float4 *>(U + 4 * nC + q) = row4(tU[4]);
-                *reinterpret_cast<float4 *>(FI + 2 * nC + q) = row4(tF[2]);
+                if (FI) *reinterpret_cast<float4 *>(FI + 2 * nC + q) = row4(tF[2]);
             }
         }
         return;
@@ -326,12 +328,14 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         if (r < nl) {
 #pragma unroll
             for (int comp = 0; comp < 4; ++comp) U[comp * nC + q] = tU[comp][r][cc];
-            FI[q] = tF[0][r][cc];
-            FI[nC + q] = tF[1][r][cc];
+            if (FI) {
+                FI[q] = tF[0][r][cc];
+                FI[nC + q] = tF[1][r][cc];
+            }
         }
         if (r <= nl) {
             U[4 * nC + q] = tU[4][r][cc];
-            FI[2 * nC + q] = tF[2][r][cc];
+            if (FI) FI[2 * nC + q] = tF[2][r][cc];
         }
     }
 }

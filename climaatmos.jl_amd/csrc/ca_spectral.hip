@@ -62,6 +62,7 @@ struct ColState {
     float sd;                  // dry static energy (vertical diffusion)
     float uc0, uc1, uc2;       // velocity in the diffusion basis
     float Kd;                  // eddy diffusivity K_h of the centre
+    float rhoe;                // rho e_tot (the final combination's U_4 value)
 };
 
 // Work item = (node, chunk of levels): chunk c owns centres and faces
@@ -77,9 +78,17 @@ struct ColState {
 // 7-level chunks (the default for large grids) the grid fills 6 per SIMD
 // almost exactly twice; at h_elem 30, 63 levels this measured 112.7 us
 // against 116.1 us for 5 per SIMD at its best chunk (6).
-template <bool SPH, bool HYPER, bool VDIFF>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH && HYPER && !VDIFF ? 6 : 1, 8)))
-k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk) {
+// FIN (the step's last explicit tendency, which also forms the final
+// combination) loads three earlier tendencies at the top of each level so
+// that they arrive during the level's arithmetic.  Those 15 values in flight
+// need registers: at 4 wavefronts per SIMD (110 VGPRs, no spill) the launch
+// measured 171 us at h_elem 30, 63 levels; at 5 (96 VGPRs, a 48-byte spill)
+// 224 us; with the loads at the store point instead, 280 us; a separate
+// tendency and combination kernel took 112 + 139 us.
+template <bool SPH, bool HYPER, bool VDIFF, bool FIN>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(SPH && HYPER && !VDIFF ? (FIN ? 4 : 6) : 1, 8)))
+k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int chunk, FinFuse fin) {
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.ncol) return;
     const Nod q = ca_node(threadIdx.x & 63);
@@ -101,7 +110,7 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
     }
     ColState prev, cur;
     prev.rhoJ = prev.irhoJ = prev.Kh = prev.Kv = prev.ct1 = prev.ct2 = prev.u1 = prev.u2 = prev.rho = prev.l4 = 0.0f;
-    prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = prev.Kd = 0.0f;
+    prev.dr = prev.d1 = prev.d2 = prev.de = prev.sd = prev.uc0 = prev.uc1 = prev.uc2 = prev.Kd = prev.rhoe = 0.0f;
     float Gp1 = 0.0f, Gp2 = 0.0f;      // G at the face below the current centre
     float Fs_prev = 0.0f, Fu0_prev = 0.0f, Fu1_prev = 0.0f, Fu2_prev = 0.0f;  // diffusive fluxes
     const int k0 = chunk > 0 ? (int)blockIdx.y * chunk : 0;
@@ -110,6 +119,19 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
     const int kend = k1;  // inclusive: face k1 closes the top owned centre
     float u3a = u3f_at(g, u3, kbeg, col);  // u_3 at face k
     for (int k = kbeg; k <= kend; ++k) {
+        // FIN: the earlier stages' tendencies at centre k - 1 and face k, loaded
+        // here so that their latency hides behind this level's arithmetic
+        float fe_c[3][4], fe_f[3];
+        if (FIN) {
+            if (k - 1 >= k0 && k - 1 < k1)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) fe_c[j][c] = ca_ldo(fin.fe[j] + c * nC + (int64_t)(k - 1) * nc, cb);
+            if ((k >= k0 && k < k1) || (k == nl && k1 == nl))
+#pragma unroll
+                for (int j = 0; j < 3; ++j) fe_f[j] = ca_ldo(fin.fe[j] + 4 * nC + (int64_t)k * nc, cb);
+        }
         float u3b = (k + 1 >= 1 && k + 1 <= nl - 1) ? ca_ldo(u3 + (int64_t)(k + 1) * nc, cb) : 0.0f;
         if (k < nl) {
             const float *Yk = Y + (int64_t)k * nc;
@@ -231,6 +253,7 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
             cur.u1 = u1;
             cur.u2 = u2;
             cur.rho = rho;
+            if (FIN) cur.rhoe = rhoe;
             cur.l4 = l4;
             cur.dr = dr;
             cur.d1 = d1;
@@ -292,7 +315,17 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
                     Fu2 = fac * (cur.uc2 - prev.uc2) * iJv;
                 }
             }
-            if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) ca_sto(Yt + 4 * nC + (int64_t)k * nc, cb, du3);
+            if ((k >= k0 && k < k1) || (k == nl && k1 == nl)) {
+                float v = du3;
+                if (FIN) {  // U_4 + dt sum_j c_j F_E,j at face k (u3a: U_4's u_3 there)
+                    float acc = fin.cg * du3;
+                    acc = fmaf(fin.c[0], fe_f[0], acc);
+                    acc = fmaf(fin.c[1], fe_f[1], acc);
+                    acc = fmaf(fin.c[2], fe_f[2], acc);
+                    v = u3a + acc;
+                }
+                ca_sto(Yt + 4 * nC + (int64_t)k * nc, cb, v);
+            }
         }
         // ---- centre k-1 is complete
         if (k - 1 >= k0 && k - 1 < k1) {
@@ -319,10 +352,22 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
                 d2 -= c2 * irho;
             }
             float *Ytk = Yt + (int64_t)kk * nc;
-            ca_sto(Ytk, cb, prev.dr);
-            ca_sto(Ytk + nC, cb, d1);
-            ca_sto(Ytk + 2 * nC, cb, d2);
-            ca_sto(Ytk + 3 * nC, cb, de);
+            if (FIN) {  // U_4 + dt sum_j c_j F_E,j at centre kk
+                const float v[4] = {prev.dr, d1, d2, de}, u[4] = {prev.rho, prev.u1, prev.u2, prev.rhoe};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float acc = fin.cg * v[c];
+                    acc = fmaf(fin.c[0], fe_c[0][c], acc);
+                    acc = fmaf(fin.c[1], fe_c[1][c], acc);
+                    acc = fmaf(fin.c[2], fe_c[2][c], acc);
+                    ca_sto(Ytk + c * nC, cb, u[c] + acc);
+                }
+            } else {
+                ca_sto(Ytk, cb, prev.dr);
+                ca_sto(Ytk + nC, cb, d1);
+                ca_sto(Ytk + 2 * nC, cb, d2);
+                ca_sto(Ytk + 3 * nC, cb, de);
+            }
         }
         Gp1 = G1;
         Gp2 = G2;
@@ -335,12 +380,30 @@ k_explicit(DevGeom g, DevNum n, const float *Y, const float *L, float *Yt, int c
     }
 }
 
-template __global__ void k_explicit<true, true, false>(DevGeom, DevNum, const float *, const float *, float *, int);
-template __global__ void k_explicit<true, false, false>(DevGeom, DevNum, const float *, const float *, float *, int);
-template __global__ void k_explicit<true, true, true>(DevGeom, DevNum, const float *, const float *, float *, int);
-template __global__ void k_explicit<true, false, true>(DevGeom, DevNum, const float *, const float *, float *, int);
-template __global__ void k_explicit<false, false, false>(DevGeom, DevNum, const float *, const float *, float *, int);
-template __global__ void k_explicit<false, false, true>(DevGeom, DevNum, const float *, const float *, float *, int);
+template __global__ void k_explicit<true, true, false, false>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, true, false, true>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, false, false, false>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, false, false, true>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, true, true, false>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, true, true, true>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, false, true, false>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<true, false, true, true>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<false, false, false, false>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<false, false, false, true>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<false, false, true, false>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
+template __global__ void k_explicit<false, false, true, true>(DevGeom, DevNum, const float *, const float *,
+                                                                    float *, int, FinFuse);
 
 // ------------------------------------------------------------ surface flux
 // surface_flux_tendency! (surface_flux.jl:101-145) with the surface conditions
@@ -360,7 +423,9 @@ template __global__ void k_explicit<false, false, true>(DevGeom, DevNum, const f
 //                   surface metric)
 // Both enter through the bottom face of a divergence with zero interior flux:
 // the tendency is the flux times J_f(surface) / J_c = s_f0^2 / (s_c0^2 dz_c0).
-__global__ void __launch_bounds__(256) k_sfc_flux(DevGeom g, DevNum n, const float *Y, float *Yt) {
+// scale multiplies what is added: 1 for the tendency, dt b_4 when Yt holds
+// the step's final combination (k_explicit with FIN).
+__global__ void __launch_bounds__(256) k_sfc_flux(DevGeom g, DevNum n, const float *Y, float *Yt, float scale) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.ncol) return;
     const CV c = cv_load(g, n, Y, 0, col);
@@ -371,7 +436,7 @@ __global__ void __launch_bounds__(256) k_sfc_flux(DevGeom g, DevNum n, const flo
     const float dz = g.zc[0] - g.zf[0];
     const float rho_s = c.rho * (T / Ts) * expf(n.grav * dz / (n.R_d * (0.5f * (T + Ts))));
     const float shf = -rho_s * n.Ch * U * (n.cp_d * (T - Ts) + n.grav * dz);
-    const float fac = s0 * s0 * g.isJc[0];
+    const float fac = scale * s0 * s0 * g.isJc[0];
     const float cD = fac * rho_s * n.Cd * U * (s0 / s) / c.rho;
     const int64_t nC = g.nC;
     Yt[nC + col] -= cD * c.u1;

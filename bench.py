@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--variants-only", action="store_true",
                     help="only the surface-flux and implicit-diffusion step timings (one JSON line)")
     ap.add_argument("--variant-steps", type=int, default=30)
+    ap.add_argument("--graphs-only", action="store_true",
+                    help="only the eager and hipGraph-replayed step at h8 z30, h16 z63 and h30 z63 (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a plain 256 MiB device-to-device copy (three launches) whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the step kernels")
@@ -359,8 +361,10 @@ def main():
         zf = G.tanh_stretched_faces(a.z_elem, 30000.0, 30.0)
         grid = G.cubed_sphere(a.h_elem, zf, par.planet_radius, par.Omega)
         out = step_variants(torch, grid, par, I.sphere_state(grid, par, seed=2), a.dt, steps=a.variant_steps)
-        print(json.dumps({"workload": f"h_elem={a.h_elem} z_elem={a.z_elem} dt={a.dt:g}s", "step_variants": out,
-                          "step_graph": step_graph_sizes(torch, par)}))
+        print(json.dumps({"workload": f"h_elem={a.h_elem} z_elem={a.z_elem} dt={a.dt:g}s", "step_variants": out}))
+        return 0
+    if a.graphs_only:
+        print(json.dumps({"step_graph": step_graph_sizes(torch, par)}))
         return 0
     if a.adjoint_only or a.ensemble_only:
         out = adjoint_c4(torch, par) if a.adjoint_only else ensemble_c3(torch, par)

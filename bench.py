@@ -138,7 +138,47 @@ def cpu_baseline(grid, params, numerics, Y0, dt, nsteps):
     }
 
 
-def ensemble_c3(torch, params):
+def cpu_baseline_c3(params, members, states, dt, nsteps):
+    """The CPU restatement (oracle/, Float64, one column per call) stepping
+    every member of the ensemble workload for nsteps, members spread over a
+    pool of host threads (each oracle context is independent; the calls
+    release the interpreter lock)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    from oracle import Oracle  # the CPU restatement: timing baseline only
+
+    from cadycore import geometry as G
+    from cadycore.params import Numerics
+
+    num = Numerics(hyperdiff=False, dt=dt, rayleigh=True, vert_diff=True)
+    work = []
+    for mem, Y0 in zip(members, states):
+        g = G.columns(1, mem.z_face, params.f_plane)
+        o = Oracle(g, params, num, alpha_w_col=np.array([mem.alpha_w]), D0_col=np.array([mem.D0]))
+        Y = np.asarray(Y0, dtype=np.float64).copy()
+        o.dss(Y)
+        o.set_precomputed(Y)
+        work.append((o, Y))
+    threads = host_threads()
+    order = sorted(range(len(work)), key=lambda m: -work[m][1].size)  # longest first
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda m: work[m][0].step(work[m][1], dt, nsteps), order))
+    el = time.perf_counter() - t0
+    return {
+        "value": round(nsteps / el, 2),
+        "unit": "ensemble steps/s",
+        "column_steps_per_s": round(nsteps * len(work) / el, 1),
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(work)} members x {nsteps} steps (all of the workload at 360), Float64, one column per call, "
+                  f"{threads} host threads: {el:.2f} s",
+    }
+
+
+def ensemble_c3(torch, params, cpu=True):
     """512 independent columns of uneven vertical sizes and per-member
     parameters, 360 ARS343 steps of 10 s in one persistent launch (after the
     autotuner picked the schedule).  The state is cache-resident (0.7 MB), so
@@ -189,6 +229,8 @@ def ensemble_c3(torch, params):
         "state_MB": round(ens.S * 4 / 1e6, 3),
     }
     ens.close()
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_c3(params, members, states, num.dt, 360)
     return out
 
 
@@ -367,7 +409,7 @@ def main():
         print(json.dumps({"step_graph": step_graph_sizes(torch, par)}))
         return 0
     if a.adjoint_only or a.ensemble_only:
-        out = adjoint_c4(torch, par) if a.adjoint_only else ensemble_c3(torch, par)
+        out = adjoint_c4(torch, par) if a.adjoint_only else ensemble_c3(torch, par, cpu=False)
         print(json.dumps(out))
         return 0
     zf = G.tanh_stretched_faces(a.z_elem, 30000.0, 30.0)
@@ -429,7 +471,7 @@ def main():
         copy_calibration(torch)
     variants = None if a.no_variants else step_variants(torch, grid, par, Y0, a.dt, steps=a.variant_steps)
     graphs = None if a.no_variants else step_graph_sizes(torch, par)
-    ens = None if a.no_ensemble else ensemble_c3(torch, par)
+    ens = None if a.no_ensemble else ensemble_c3(torch, par, cpu=not a.no_cpu_baseline)
     adj = None if a.no_adjoint else adjoint_c4(torch, par)
 
     cpu = None

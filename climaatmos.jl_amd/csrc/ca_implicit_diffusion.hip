@@ -293,8 +293,10 @@ __device__ __forceinline__ float solve_f32(float L, float D, float U, float r) {
     for (int s = 1; s < 64; s <<= 1) {
         const float iD = 1.0f / D;
         const float Ln = L * iD, Un = U * iD, rn = r * iD;
-        const float La = pcr::up(Ln, s), Ua = pcr::up(Un, s), ra = pcr::up(rn, s);
-        const float Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), rb = pcr::dn(rn, s);
+        float La, Ua, ra, Lb, Ub, rb;
+        pcr::pcr_nbrs(Ln, s, La, Lb);
+        pcr::pcr_nbrs(Un, s, Ua, Ub);
+        pcr::pcr_nbrs(rn, s, ra, rb);
         const float nD = D - L * Ua - U * Lb;
         const float nr = r - L * ra - U * rb;
         L = -L * La;
@@ -310,8 +312,11 @@ __device__ __forceinline__ void solve2_f32(float L, float D, float U, float &r1,
     for (int s = 1; s < 64; s <<= 1) {
         const float iD = 1.0f / D;
         const float Ln = L * iD, Un = U * iD, an = r1 * iD, bn = r2 * iD;
-        const float La = pcr::up(Ln, s), Ua = pcr::up(Un, s), aa = pcr::up(an, s), ba = pcr::up(bn, s);
-        const float Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), ab = pcr::dn(an, s), bb = pcr::dn(bn, s);
+        float La, Ua, aa, ba, Lb, Ub, ab, bb;
+        pcr::pcr_nbrs(Ln, s, La, Lb);
+        pcr::pcr_nbrs(Un, s, Ua, Ub);
+        pcr::pcr_nbrs(an, s, aa, ab);
+        pcr::pcr_nbrs(bn, s, ba, bb);
         const float nD = D - L * Ua - U * Lb;
         r1 = r1 - L * aa - U * ab;
         r2 = r2 - L * ba - U * bb;
@@ -328,8 +333,10 @@ __device__ __forceinline__ float solve_f64(double Ld, double Dd, double Ud, doub
     for (int s = 1; s < 64; s <<= 1) {
         const double iD = ca_drcp(Dd);
         const double Ln = Ld * iD, Un = Ud * iD, rn = rd * iD;
-        const double La = pcr::up(Ln, s), Ua = pcr::up(Un, s), ra = pcr::up(rn, s);
-        const double Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), rb = pcr::dn(rn, s);
+        double La, Ua, ra, Lb, Ub, rb;
+        pcr::pcr_nbrs(Ln, s, La, Lb);
+        pcr::pcr_nbrs(Un, s, Ua, Ub);
+        pcr::pcr_nbrs(rn, s, ra, rb);
         const double nD = Dd - Ld * Ua - Ud * Lb;
         const double nr = rd - Ld * ra - Ud * rb;
         Ld = -Ld * La;
@@ -363,9 +370,9 @@ __device__ __forceinline__ Sol arrowhead_solve(int k, int nl, bool hasC, bool in
                                                const FaceRow &fr, float bw0, int iters) {
     // y_e = inv(A_ee) b_e; b_w' = b_w - A_wrho y_rho - A_we y_e (y_rho = -b_rho)
     const float ye = solve_f32(c.elo, c.ed, c.eup, c.be);
-    const float pye = pcr::up(ye, 1), pbr = pcr::up(c.br, 1);
-    const float pa0 = pcr::up(c.a0, 1), pa1 = pcr::up(c.a1, 1), pe0 = pcr::up(c.e0, 1), pe1 = pcr::up(c.e1, 1);
-    const float ped = pcr::up(c.ed, 1);
+    const float pye = pcr::up1(ye), pbr = pcr::up1(c.br);
+    const float pa0 = pcr::up1(c.a0), pa1 = pcr::up1(c.a1), pe0 = pcr::up1(c.e0), pe1 = pcr::up1(c.e1);
+    const float ped = pcr::up1(c.ed);
     float bw = bw0, Pl = fr.lo, Pd = fr.d, Pu = fr.up;
     if (intF) {
         bw = bw0 + fr.r0 * pbr + fr.r1 * c.br - (fr.e0 * pye + fr.e1 * ye);
@@ -383,18 +390,18 @@ __device__ __forceinline__ Sol arrowhead_solve(int k, int nl, bool hasC, bool in
     for (int it = 0; it < iters; ++it) {
         float rh1 = c.bh1, rh2 = c.bh2, rw = bw;
         if (it > 0) {  // r = b' - S x
-            const float x1m = pcr::up(xh1, 1), x1p = pcr::dn(xh1, 1);
-            const float x2m = pcr::up(xh2, 1), x2p = pcr::dn(xh2, 1);
-            const float xwm = pcr::up(xw, 1), xwp = pcr::dn(xw, 1);
+            const float x1m = pcr::up1(xh1), x1p = pcr::dn1(xh1);
+            const float x2m = pcr::up1(xh2), x2p = pcr::dn1(xh2);
+            const float xwm = pcr::up1(xw), xwp = pcr::dn1(xw);
             if (hasC) {
                 rh1 -= c.hd * xh1 + (k >= 1 ? c.hlo * x1m : 0.0f) + (k + 1 < nl ? c.hup * x1p : 0.0f);
                 rh2 -= c.hd * xh2 + (k >= 1 ? c.hlo * x2m : 0.0f) + (k + 1 < nl ? c.hup * x2p : 0.0f);
             }
             const float gg = hasC ? c.e0 * xw + c.e1 * xwp : 0.0f;  // A_ew x_w
             const float mm = solve_f32(c.elo, c.ed, c.eup, gg);
-            const float pmm = pcr::up(mm, 1);
+            const float pmm = pcr::up1(mm);
             const float pA = c.a0 * xw + c.a1 * xwp;  // A_rho,w x_w at centre k
-            const float ppA = pcr::up(pA, 1);
+            const float ppA = pcr::up1(pA);
             float sx = fr.d * xw + (k >= 1 ? fr.lo * xwm : 0.0f) + (k + 1 <= nl ? fr.up * xwp : 0.0f);
             if (intF) {
                 sx += fr.h0a * x1m + fr.h0b * x2m + fr.h1a * xh1 + fr.h1b * xh2;
@@ -405,7 +412,7 @@ __device__ __forceinline__ Sol arrowhead_solve(int k, int nl, bool hasC, bool in
         }
         // z_h = inv(A_hh) r_h (two components); z_w = inv(P_ww) (r_w - A_wh z_h)
         solve2_f32(c.hlo, c.hd, c.hup, rh1, rh2);
-        const float z1m = pcr::up(rh1, 1), z2m = pcr::up(rh2, 1);
+        const float z1m = pcr::up1(rh1), z2m = pcr::up1(rh2);
         if (intF) rw -= fr.h0a * z1m + fr.h0b * z2m + fr.h1a * rh1 + fr.h1b * rh2;
         const float zw = solve_f64(Pl, Pd, Pu, rw);
         if (hasC) {
@@ -414,7 +421,7 @@ __device__ __forceinline__ Sol arrowhead_solve(int k, int nl, bool hasC, bool in
         }
         if (isF) xw += zw;
     }
-    const float xwp = pcr::dn(xw, 1);
+    const float xwp = pcr::dn1(xw);
     const float gg = hasC ? c.be - (c.e0 * xw + c.e1 * xwp) : 0.0f;
     Sol x;
     x.xe = solve_f32(c.elo, c.ed, c.eup, gg);
@@ -510,12 +517,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         if (col >= nc) break;  // uniform across the wavefront
         const float rho = tU[0][kc][cc], u1 = tU[1][kc][cc], u2 = tU[2][kc][cc], rhoe = tU[3][kc][cc];
         const float u3 = intF ? tU[4][k][cc] : 0.0f;
-        const float u3p = pcr::dn(u3, 1);
-        const float u3m = pcr::up(u3, 1);
+        const float u3p = pcr::dn1(u3);
+        const float u3m = pcr::up1(u3);
         const CV c = cv_make(g, n, kc, col, rho, u1, u2, rhoe, u3, hasC ? u3p : 0.0f);
         const float rJ = c.rho * c.J;
-        const float rJm = pcr::up(rJ, 1), hm = pcr::up(c.h, 1);
-        const float rJp = pcr::dn(rJ, 1), hp = pcr::dn(c.h, 1);
+        const float rJm = pcr::up1(rJ), hm = pcr::up1(c.h);
+        const float rJp = pcr::dn1(rJ), hp = pcr::dn1(c.h);
         // centre row k: A(rho,u3), A(rhoe,u3) and the advective residuals
         float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f, Rr = 0.0f, Te = 0.0f;
         if (hasC) {
@@ -539,26 +546,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         }
         // centre k - 1 (lane k - 1)
         CV a;
-        a.rho = pcr::up(c.rho, 1);
-        a.u1 = pcr::up(c.u1, 1);
-        a.u2 = pcr::up(c.u2, 1);
-        a.K = pcr::up(c.K, 1);
-        a.Td = pcr::up(c.Td, 1);
-        a.pd = pcr::up(c.pd, 1);
-        a.Pid = pcr::up(c.Pid, 1);
+        a.rho = pcr::up1(c.rho);
+        a.u1 = pcr::up1(c.u1);
+        a.u2 = pcr::up1(c.u2);
+        a.K = pcr::up1(c.K);
+        a.Td = pcr::up1(c.Td);
+        a.pd = pcr::up1(c.pd);
+        a.Pid = pcr::up1(c.Pid);
         a.T = (float)a.Td;
         a.p = (float)a.pd;
         a.Pi = (float)a.Pid;
-        a.ct1 = pcr::up(c.ct1, 1);
-        a.ct2 = pcr::up(c.ct2, 1);
+        a.ct1 = pcr::up1(c.ct1);
+        a.ct2 = pcr::up1(c.ct2);
         // vertical diffusion: fluxes through face k, the tendency of centre k
         // the centre's diffusivity, and the lowest centre's wind speed (lane 0)
         const float spd0 = n.vdk == 1 ? __shfl(sqrtf(c.u1 * c.ct1 + c.u2 * c.ct2), 0, 64) : 0.0f;
         const float Kd = ca_Kc(g, n, col, kc, c.p, spd0);
-        const float Kdm = pcr::up(Kd, 1), Kdp = pcr::dn(Kd, 1);
+        const float Kdm = pcr::up1(Kd), Kdp = pcr::dn1(Kd);
         float Fs = 0.0f, Fu0 = 0.0f, Fu1 = 0.0f, Fu2 = 0.0f;
         if (intF) vdiff_face(g, n, col, k, a, c, Kdm, Kd, Fs, Fu0, Fu1, Fu2);
-        const float Fs1 = pcr::dn(Fs, 1), Fu01 = pcr::dn(Fu0, 1), Fu11 = pcr::dn(Fu1, 1), Fu21 = pcr::dn(Fu2, 1);
+        const float Fs1 = pcr::dn1(Fs), Fu01 = pcr::dn1(Fu0), Fu11 = pcr::dn1(Fu1), Fu21 = pcr::dn1(Fu2);
         float Rh1 = 0.0f, Rh2 = 0.0f;
         if (hasC) {
             const float iJc = c.iJ;
@@ -581,7 +588,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         // diffusion blocks of centre k (rows >= nl: identity)
         float elo = 0.0f, ed = 1.0f, eup = 0.0f, hlo = 0.0f, hd = 1.0f, hup = 0.0f;
         if (hasC) {
-            const float rm = a.rho, rp = pcr::dn(c.rho, 1), r0 = c.rho;
+            const float rm = a.rho, rp = pcr::dn1(c.rho), r0 = c.rho;
             const float Jc = c.J;
             float dlo = 0.0f, dup = 0.0f;
             if (k >= 1) {
@@ -636,14 +643,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         const float u3_n = intF ? u3 - xw : 0.0f;
         // post-Newton upwind correction at the new state
         if (n.upw != 0) {
-            const float u3_np = pcr::dn(u3_n, 1);
+            const float u3_np = pcr::dn1(u3_n);
             const float hn = cv_enthalpy(g, n, kc, col, rho_n, u1_n, u2_n, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
             const float href = __shfl(hn, 0, 64);
             const float hr = hn - href;
             const float sk = g.sc[kc];
             const float rJn = rho_n * (g.Jh[col] * sk * sk * g.Jvc[kc]);
-            const float h2 = pcr::up(hr, 2), h1 = pcr::up(hr, 1), h4 = pcr::dn(hr, 1);
-            const float rJn1 = pcr::up(rJn, 1);
+            const float h2 = pcr::up(hr, 2), h1 = pcr::up1(hr), h4 = pcr::dn1(hr);
+            const float rJn1 = pcr::up1(rJn);
             float dF = 0.0f;
             if (intF) {
                 const float v = u3_n * g33;
@@ -652,7 +659,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
                 const float cen = upwind_product(0, k, nl, v, n.dt, 0.0f, h1, hr, 0.0f);
                 dF = rJf * upw - rJf * cen;
             }
-            const float dF1 = pcr::dn(dF, 1);
+            const float dF1 = pcr::dn1(dF);
             const float corr = hasC ? -(dF1 - dF) * (g.iJh[col] * g.isJc[kc]) : 0.0f;
             rhoe_n += dtg * corr;
         }

@@ -21,6 +21,70 @@ __device__ __forceinline__ T up(T v, int d) { return __shfl_up(v, d, 64); }
 template <class T>
 __device__ __forceinline__ T dn(T v, int d) { return __shfl_down(v, d, 64); }
 
+// Shifts by one lane across the whole wavefront as DPP wave shifts
+// (wave_shr:1, lane i <- lane i - 1; wave_shl:1, lane i <- lane i + 1) in the
+// operand of a VALU move instead of an LDS-crossbar permute.  The edge lane
+// has no source and keeps its own value (old = v, bound_ctrl off), as
+// __shfl_up / __shfl_down leave an out-of-range lane.  All lanes are active
+// wherever these are called (uniform control flow).
+__device__ __forceinline__ int up1i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int dn1i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ float up1(float v) { return __int_as_float(up1i(__float_as_int(v))); }
+__device__ __forceinline__ float dn1(float v) { return __int_as_float(dn1i(__float_as_int(v))); }
+__device__ __forceinline__ double up1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = up1i((int)(b & 0xffffffffll)), hi = up1i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double dn1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dn1i((int)(b & 0xffffffffll)), hi = dn1i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Both neighbours at distance 32 from one v_permlane32_swap of v with itself:
+// the swap exchanges lanes 32-63 of its first operand with lanes 0-31 of its
+// second, so the first result holds lane i - 32 in lanes 32-63 (own value
+// below) and the second lane i + 32 in lanes 0-31 (own value above): exactly
+// __shfl_up(v, 32) and __shfl_down(v, 32).
+__device__ __forceinline__ void swap32(double v, double &u, double &d) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    u = __longlong_as_double(((long long)h[0] << 32) | l[0]);
+    d = __longlong_as_double(((long long)h[1] << 32) | l[1]);
+}
+
+__device__ __forceinline__ void swap32(float v, float &u, float &d) {
+    const unsigned b = __float_as_uint(v);
+    const auto r = __builtin_amdgcn_permlane32_swap(b, b, false, false);
+    u = __uint_as_float(r[0]);
+    d = __uint_as_float(r[1]);
+}
+
+// The rows k - s (u) and k + s (d) of a parallel-cyclic-reduction step, s a
+// power of two below 64 that is a constant once the step loop is unrolled.
+// s = 1: DPP wave shifts; s = 2: two of them; s = 32: one lane swap per
+// dword; otherwise LDS-crossbar permutes.  Out-of-range lanes receive some
+// lane's finite value (with s = 2 lane 1 receives lane 0's, not its own):
+// PCR multiplies every such value by an exactly zero coupling.
+template <class T>
+__device__ __forceinline__ void pcr_nbrs(T v, int s, T &u, T &d) {
+    if (s == 1) {
+        u = up1(v);
+        d = dn1(v);
+    } else if (s == 2) {
+        u = up1(up1(v));
+        d = dn1(dn1(v));
+    } else if (s == 32) {
+        swap32(v, u, d);
+    } else {
+        u = up(v, s);
+        d = dn(v, s);
+    }
+}
+
 }  // namespace pcr
 
 // NF / NI: compile-time numbers of full-state and implicit-tendency sources
@@ -174,13 +238,13 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         if (col >= nc) break;  // uniform across the wavefront
         const float rho = tU[0][kc][cc], u1 = tU[1][kc][cc], u2 = tU[2][kc][cc], rhoe = tU[3][kc][cc];
         const float u3 = intF ? tU[4][k][cc] : 0.0f;
-        const float u3p = pcr::dn(u3, 1);  // face k + 1
-        const float u3m = pcr::up(u3, 1);  // face k - 1
+        const float u3p = pcr::dn1(u3);  // face k + 1
+        const float u3m = pcr::up1(u3);  // face k - 1
         const CV c = cv_make(g, n, kc, col, rho, u1, u2, rhoe, u3, hasC ? u3p : 0.0f);
         const float rJ = c.rho * c.J;
         // centre row k: A(rho,u3), A(rhoe,u3) on faces k, k+1 and R = dtg T_imp
-        const float rJm = pcr::up(rJ, 1), hm = pcr::up(c.h, 1);
-        const float rJp = pcr::dn(rJ, 1), hp = pcr::dn(c.h, 1);
+        const float rJm = pcr::up1(rJ), hm = pcr::up1(c.h);
+        const float rJp = pcr::dn1(rJ), hp = pcr::dn1(c.h);
         float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f, Rr = 0.0f, Re = 0.0f;
         if (hasC) {
             float Fm = 0.0f, Fp = 0.0f, Hm = 0.0f, Hp = 0.0f;
@@ -203,18 +267,18 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         }
         // face row k: needs centre k - 1 (lane k - 1) and centre k (own lane)
         CV a;
-        a.rho = pcr::up(c.rho, 1);
-        a.K = pcr::up(c.K, 1);
-        a.Td = pcr::up(c.Td, 1);
-        a.pd = pcr::up(c.pd, 1);
-        a.Pid = pcr::up(c.Pid, 1);
+        a.rho = pcr::up1(c.rho);
+        a.K = pcr::up1(c.K);
+        a.Td = pcr::up1(c.Td);
+        a.pd = pcr::up1(c.pd);
+        a.Pid = pcr::up1(c.Pid);
         a.T = (float)a.Td;
         a.p = (float)a.pd;
         a.Pi = (float)a.Pid;
-        a.ct1 = pcr::up(c.ct1, 1);
-        a.ct2 = pcr::up(c.ct2, 1);
-        const float pa0 = pcr::up(a0, 1), pa1 = pcr::up(a1, 1), pe0 = pcr::up(e0, 1), pe1 = pcr::up(e1, 1);
-        const float pRr = pcr::up(Rr, 1), pRe = pcr::up(Re, 1);
+        a.ct1 = pcr::up1(c.ct1);
+        a.ct2 = pcr::up1(c.ct2);
+        const float pa0 = pcr::up1(a0), pa1 = pcr::up1(a1), pe0 = pcr::up1(e0), pe1 = pcr::up1(e1);
+        const float pRr = pcr::up1(Rr), pRe = pcr::up1(Re);
         float L = 0.0f, D = 1.0f, Up = 0.0f, rhs = 0.0f;
         if (intF) {
             const FaceRow fr = jac_face_row(g, n, k, col, a, c, u3m, u3, u3p, dtg);
@@ -234,15 +298,18 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         // (L, U, r) / D and forms
         //   L' = -L La~,  U' = -U Ub~,  D' = D - L Ua~ - U Lb~,  r' = r - L ra~ - U rb~.
         // Row k couples only to rows k -+ s after step s / 2, so for k < s
-        // (k + s > 63) its L (U) is exactly 0, and the out-of-range shuffles,
-        // which return the lane's own finite values, are multiplied by 0.
+        // (k + s > 63) its L (U) is exactly 0, and the out-of-range
+        // exchanges, which return some lane's finite values, are multiplied
+        // by 0 (pcr::pcr_nbrs).
         double Ld = L, Dd = D, Ud = Up, rd = rhs;
 #pragma unroll
         for (int s = 1; s < 64; s <<= 1) {
             const double iD = ca_drcp(Dd);
             const double Ln = Ld * iD, Un = Ud * iD, rn = rd * iD;
-            const double La = pcr::up(Ln, s), Ua = pcr::up(Un, s), ra = pcr::up(rn, s);
-            const double Lb = pcr::dn(Ln, s), Ub = pcr::dn(Un, s), rb = pcr::dn(rn, s);
+            double La, Ua, ra, Lb, Ub, rb;
+            pcr::pcr_nbrs(Ln, s, La, Lb);
+            pcr::pcr_nbrs(Un, s, Ua, Ub);
+            pcr::pcr_nbrs(rn, s, ra, rb);
             const double nD = Dd - Ld * Ua - Ud * Lb;
             const double nr = rd - Ld * ra - Ud * rb;
             Ld = -Ld * La;
@@ -251,7 +318,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
             rd = nr;
         }
         const float x = (float)(rd * ca_drcp(Dd));  // du_3 at face k
-        const float x1 = pcr::dn(x, 1);   // du_3 at face k + 1
+        const float x1 = pcr::dn1(x);   // du_3 at face k + 1
         // Newton update
         const float dr = a0 * x + a1 * x1 - Rr;
         const float de = e0 * x + e1 * x1 - Re;
@@ -261,14 +328,14 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         float fi_e = -de * idtg;
         // post-Newton upwind correction at the new state
         if (n.upw != 0) {
-            const float u3_np = pcr::dn(u3_n, 1);
+            const float u3_np = pcr::dn1(u3_n);
             const float hn = cv_enthalpy(g, n, kc, col, rho_n, u1, u2, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
             const float href = __shfl(hn, 0, 64);
             const float hr = hn - href;
             const float sk = g.sc[kc];
             const float rJn = rho_n * (g.Jh[col] * sk * sk * g.Jvc[kc]);  // J: geometry only, reloaded
-            const float h2 = pcr::up(hr, 2), h1 = pcr::up(hr, 1), h4 = pcr::dn(hr, 1);
-            const float rJn1 = pcr::up(rJn, 1);
+            const float h2 = pcr::up(hr, 2), h1 = pcr::up1(hr), h4 = pcr::dn1(hr);
+            const float rJn1 = pcr::up1(rJn);
             float dF = 0.0f;
             if (intF) {
                 const float v = u3_n * g33;
@@ -277,7 +344,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
                 const float cen = upwind_product(0, k, nl, v, n.dt, 0.0f, h1, hr, 0.0f);
                 dF = rJf * upw - rJf * cen;
             }
-            const float dF1 = pcr::dn(dF, 1);
+            const float dF1 = pcr::dn1(dF);
             const float corr = hasC ? -(dF1 - dF) * (g.iJh[col] * g.isJc[kc]) : 0.0f;
             rhoe_n += dtg * corr;
             fi_e += corr;

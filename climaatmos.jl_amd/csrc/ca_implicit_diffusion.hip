@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         a.ct2 = pcr::up1(c.ct2);
         // vertical diffusion: fluxes through face k, the tendency of centre k
         // the centre's diffusivity, and the lowest centre's wind speed (lane 0)
-        const float spd0 = n.vdk == 1 ? __shfl(sqrtf(c.u1 * c.ct1 + c.u2 * c.ct2), 0, 64) : 0.0f;
+        const float spd0 = n.vdk == 1 ? pcr::lane0(sqrtf(c.u1 * c.ct1 + c.u2 * c.ct2)) : 0.0f;
         const float Kd = ca_Kc(g, n, col, kc, c.p, spd0);
         const float Kdm = pcr::up1(Kd), Kdp = pcr::dn1(Kd);
         float Fs = 0.0f, Fu0 = 0.0f, Fu1 = 0.0f, Fu2 = 0.0f;
@@ -645,7 +645,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
         if (n.upw != 0) {
             const float u3_np = pcr::dn1(u3_n);
             const float hn = cv_enthalpy(g, n, kc, col, rho_n, u1_n, u2_n, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
-            const float href = __shfl(hn, 0, 64);
+            const float href = pcr::lane0(hn);
             const float hr = hn - href;
             const float sk = g.sc[kc];
             const float rJn = rho_n * (g.Jh[col] * sk * sk * g.Jvc[kc]);

@@ -42,6 +42,9 @@ __device__ __forceinline__ double dn1(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// lane 0's value in every lane (v_readlane_b32 into a scalar register)
+__device__ __forceinline__ float lane0(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)); }
+
 // Both neighbours at distance 32 from one v_permlane32_swap of v with itself:
 // the swap exchanges lanes 32-63 of its first operand with lanes 0-31 of its
 // second, so the first result holds lane i - 32 in lanes 32-63 (own value
@@ -330,7 +333,7 @@ __global__ void __launch_bounds__(256) k_newton_pcr(DevGeom g, DevNum n, StageSr
         if (n.upw != 0) {
             const float u3_np = pcr::dn1(u3_n);
             const float hn = cv_enthalpy(g, n, kc, col, rho_n, u1, u2, rhoe_n, u3_n, hasC ? u3_np : 0.0f);
-            const float href = __shfl(hn, 0, 64);
+            const float href = pcr::lane0(hn);
             const float hr = hn - href;
             const float sk = g.sc[kc];
             const float rJn = rho_n * (g.Jh[col] * sk * sk * g.Jvc[kc]);  // J: geometry only, reloaded

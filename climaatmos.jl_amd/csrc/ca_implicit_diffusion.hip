@@ -285,13 +285,22 @@ __global__ void __launch_bounds__(128) k_ldiv_approx(DevGeom g, Bands A, DiffBan
 
 namespace idp {
 
+// 1 / d from the hardware reciprocal (1 ulp) and one Newton-Raphson step
+// r' = r + r (1 - d r): three VALU instructions instead of the correctly
+// rounded division sequence.  The PCR diagonals are O(1) (-I minus a
+// diffusion operator), far from the denormal range.
+__device__ __forceinline__ float rcp_nr(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+
 // PCR on the 64 lane rows (row k couples to rows k -+ 1 through L and U;
 // L = 0 on row 0 and U = 0 on row 63): step s eliminates the couplings to
 // rows k -+ s with the neighbour rows divided by their own diagonal.
 __device__ __forceinline__ float solve_f32(float L, float D, float U, float r) {
 #pragma unroll
     for (int s = 1; s < 64; s <<= 1) {
-        const float iD = 1.0f / D;
+        const float iD = rcp_nr(D);
         const float Ln = L * iD, Un = U * iD, rn = r * iD;
         float La, Ua, ra, Lb, Ub, rb;
         pcr::pcr_nbrs(Ln, s, La, Lb);
@@ -304,13 +313,13 @@ __device__ __forceinline__ float solve_f32(float L, float D, float U, float r) {
         D = nD;
         r = nr;
     }
-    return r / D;
+    return r * rcp_nr(D);
 }
 // two right-hand sides of one matrix
 __device__ __forceinline__ void solve2_f32(float L, float D, float U, float &r1, float &r2) {
 #pragma unroll
     for (int s = 1; s < 64; s <<= 1) {
-        const float iD = 1.0f / D;
+        const float iD = rcp_nr(D);
         const float Ln = L * iD, Un = U * iD, an = r1 * iD, bn = r2 * iD;
         float La, Ua, aa, ba, Lb, Ub, ab, bb;
         pcr::pcr_nbrs(Ln, s, La, Lb);
@@ -324,7 +333,7 @@ __device__ __forceinline__ void solve2_f32(float L, float D, float U, float &r1,
         U = -U * Ub;
         D = nD;
     }
-    const float iD = 1.0f / D;
+    const float iD = rcp_nr(D);
     r1 *= iD;
     r2 *= iD;
 }

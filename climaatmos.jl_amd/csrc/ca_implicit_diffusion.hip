@@ -385,9 +385,10 @@ __device__ __forceinline__ Sol arrowhead_solve(int k, int nl, bool hasC, bool in
     float bw = bw0, Pl = fr.lo, Pd = fr.d, Pu = fr.up;
     if (intF) {
         bw = bw0 + fr.r0 * pbr + fr.r1 * c.br - (fr.e0 * pye + fr.e1 * ye);
-        Pl = fr.lo + fr.r0 * pa0 - fr.e0 * pe0 / ped;
-        Pd = fr.d + fr.r0 * pa1 + fr.r1 * c.a0 - (fr.e0 * pe1 / ped + fr.e1 * c.e0 / c.ed);
-        Pu = fr.up + fr.r1 * c.a1 - fr.e1 * c.e1 / c.ed;
+        const float iped = rcp_nr(ped), ied = rcp_nr(c.ed);
+        Pl = fr.lo + fr.r0 * pa0 - fr.e0 * pe0 * iped;
+        Pd = fr.d + fr.r0 * pa1 + fr.r1 * c.a0 - (fr.e0 * pe1 * iped + fr.e1 * c.e0 * ied);
+        Pu = fr.up + fr.r1 * c.a1 - fr.e1 * c.e1 * ied;
     }
     if (!isF) {
         Pl = 0.0f;
@@ -521,6 +522,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
     const float g33 = g.g33f[kf];
     const float g33p = g.g33f[kc + 1];
     const float kcv = n.cp_d / n.cv_d;
+    const float idtg = 1.0f / dtg;
     for (int cc = wid; cc < TC; cc += 4) {
         const int col = col0 + cc;
         if (col >= nc) break;  // uniform across the wavefront
@@ -590,31 +592,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
                 c1 = d0;
                 c2 = d1;
             }
-            Rh1 = dtg * (0.0f - c1 / c.rho);
-            Rh2 = dtg * (0.0f - c2 / c.rho);
+            const float irho = idp::rcp_nr(c.rho);
+            Rh1 = dtg * (0.0f - c1 * irho);
+            Rh2 = dtg * (0.0f - c2 * irho);
         }
         const float Re = dtg * Te;
         // diffusion blocks of centre k (rows >= nl: identity)
         float elo = 0.0f, ed = 1.0f, eup = 0.0f, hlo = 0.0f, hd = 1.0f, hup = 0.0f;
         if (hasC) {
             const float rm = a.rho, rp = pcr::dn1(c.rho), r0 = c.rho;
-            const float Jc = c.J;
+            const float iJc = idp::rcp_nr(c.J);
+            const float irm = idp::rcp_nr(rm), ir0 = idp::rcp_nr(r0), irp = idp::rcp_nr(rp);
             float dlo = 0.0f, dup = 0.0f;
             if (k >= 1) {
                 const float sf = g.sf[k];
-                dlo = (g.Jh[col] * sf * sf * g.Jvf[k]) * g.g33f[k] * ca_rhoK(rm, r0, Kdm, Kd) / Jc;
+                dlo = (g.Jh[col] * sf * sf * g.Jvf[k]) * g.g33f[k] * ca_rhoK(rm, r0, Kdm, Kd) * iJc;
             }
             if (k + 1 <= nl - 1) {
                 const float sf = g.sf[k + 1];
-                dup = (g.Jh[col] * sf * sf * g.Jvf[k + 1]) * g.g33f[k + 1] * ca_rhoK(r0, rp, Kd, Kdp) / Jc;
+                dup = (g.Jh[col] * sf * sf * g.Jvf[k + 1]) * g.g33f[k + 1] * ca_rhoK(r0, rp, Kd, Kdp) * iJc;
             }
             const float dd = -(dlo + dup);
-            elo = k >= 1 ? dtg * dlo * (kcv / rm) : 0.0f;
-            ed = dtg * dd * (kcv / r0) - 1.0f;
-            eup = k + 1 <= nl - 1 ? dtg * dup * (kcv / rp) : 0.0f;
-            hlo = dtg * (1.0f / r0) * dlo;
-            hd = dtg * (1.0f / r0) * dd - 1.0f;
-            hup = dtg * (1.0f / r0) * dup;
+            elo = k >= 1 ? dtg * dlo * (kcv * irm) : 0.0f;
+            ed = dtg * dd * (kcv * ir0) - 1.0f;
+            eup = k + 1 <= nl - 1 ? dtg * dup * (kcv * irp) : 0.0f;
+            hlo = dtg * ir0 * dlo;
+            hd = dtg * ir0 * dd - 1.0f;
+            hup = dtg * ir0 * dup;
         }
         // face row k and its residual
         FaceRow fr;
@@ -678,14 +682,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
             tU[1][k][cc] = u1_n;
             tU[2][k][cc] = u2_n;
             tU[3][k][cc] = rhoe_n;
-            tF[0][k][cc] = (rho_n - rho) / dtg;
-            tF[1][k][cc] = (u1_n - u1) / dtg;
-            tF[2][k][cc] = (u2_n - u2) / dtg;
-            tF[3][k][cc] = (rhoe_n - rhoe) / dtg;
+            tF[0][k][cc] = (rho_n - rho) * idtg;
+            tF[1][k][cc] = (u1_n - u1) * idtg;
+            tF[2][k][cc] = (u2_n - u2) * idtg;
+            tF[3][k][cc] = (rhoe_n - rhoe) * idtg;
         }
         if (isF) {
             tU[4][k][cc] = u3_n;
-            tF[4][k][cc] = (u3_n - u3) / dtg;
+            tF[4][k][cc] = (u3_n - u3) * idtg;
         }
     }
     __syncthreads();

@@ -532,6 +532,40 @@ def test_ars343_tableau_order_conditions_and_convergence():
     assert 2.8 < math.log2(e1 / e2) < 3.3
 
 
+def test_ars343_final_combination_from_the_last_stage():
+    """ARS343's implicit tableau is stiffly accurate (b = last row of a_I), so
+    with F_I back-computed as (U - U~) / (dt a_I[i][i]) the step's final
+    combination u + dt sum b_j (F_E,j + F_I,j) equals U_4 + dt sum_j (b_j -
+    a_E[3][j]) F_E,j, with a_E[3][3] = 0: the form in which the native step
+    forms it inside its last explicit tendency.  Checked on a nonlinear
+    vector system (not only y' = lam y), and by a plausible slip failing it
+    (dropping the a_E[3][j] terms)."""
+    aE, aI, b, c = tableau()
+    np.testing.assert_array_equal(aI[3], b)
+    assert aE[3, 3] == 0.0 and aI[0].sum() == 0.0
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((6, 6)) * 0.3
+    lin = -np.diag(rng.uniform(0.5, 3.0, 6))
+
+    def fe(y):  # explicit tendency: nonlinear
+        return np.tanh(M @ y) + 0.1 * y ** 2
+
+    dt, u = 0.2, rng.standard_normal(6)
+    FE, FI, U = [None] * 4, [None] * 4, [u]
+    FE[0] = fe(u)
+    for i in range(1, 4):
+        ut = u + dt * sum(aE[i, j] * FE[j] for j in range(i)) + dt * sum(aI[i, j] * FI[j] for j in range(1, i))
+        Ui = np.linalg.solve(np.eye(6) - dt * aI[i, i] * lin, ut)  # the implicit solve
+        FI[i] = (Ui - ut) / (dt * aI[i, i])
+        FE[i] = fe(Ui)
+        U.append(Ui)
+    ref = u + dt * sum(b[j] * (FE[j] + (FI[j] if j else 0)) for j in range(4))
+    fused = U[3] + dt * sum((b[j] - aE[3, j]) * FE[j] for j in range(4))
+    np.testing.assert_allclose(fused, ref, rtol=1e-13, atol=1e-13)
+    slip = U[3] + dt * sum(b[j] * FE[j] for j in range(4))
+    assert np.abs(slip - ref).max() > 1e-3
+
+
 def test_rayleigh_sponge_profile_closed_form():
     """beta = alpha sin^2(pi (z - zd) / (2 (zmax - zd))) above zd
     (rayleigh_sponge.jl:7-28, sponge.jl test)."""

@@ -53,7 +53,7 @@ __device__ __forceinline__ Cen cen_eval(const DevGeom &g, const DevNum &n, int k
     c.ct1 = is2 * ((double)g.gu11[0] * u1 + (double)g.gu12[0] * u2);
     c.ct2 = is2 * ((double)g.gu12[0] * u1 + (double)g.gu22[0] * u2);
     c.K = 0.5 * ((u1 * c.ct1 + u2 * c.ct2) + 0.5 * ((double)g.g33f[k] * (u3a * u3a) + (double)g.g33f[k + 1] * (u3b * u3b)));
-    c.e = re / rho;
+    c.e = re * ca_drcp(rho);  // as the forward forms e_tot
     c.T = Td;
     c.free_T = Td > n.dTmin;  // the clamp T_min sets Td to exactly T_min
     c.p = pd;
@@ -63,14 +63,15 @@ __device__ __forceinline__ Cen cen_eval(const DevGeom &g, const DevNum &n, int k
 // reverse of cen_eval: cotangents of T, p (direct) and Pi -> rho, u1, u2, rho e_tot, u3a, u3b
 __device__ __forceinline__ void cen_reverse(const DevGeom &g, const DevNum &n, int k, const Cen &c, double u3a,
                                             double u3b, double Tb, double pb, double Pib, double Kb, double *out) {
-    pb += Pib * n.dkex * c.Pi / c.p;
+    pb += Pib * n.dkex * c.Pi * ca_drcp(c.p);
     double rhob = n.dR_d * c.T * pb;
     Tb += n.dR_d * c.rho * pb;
     const double Tub = c.free_T ? Tb : 0.0;
     const double eb = n.dicv_d * Tub;
     Kb -= n.dicv_d * Tub;
-    out[3] += eb / c.rho;
-    rhob -= eb * c.e / c.rho;
+    const double irho = ca_drcp(c.rho);
+    out[3] += eb * irho;
+    rhob -= eb * c.e * irho;
     out[0] += rhob;
     // K = (u . ct) / 2 + (g_a u3a^2 + g_b u3b^2) / 4, ct = is2 G^-1 u (symmetric)
     out[1] += Kb * c.ct1;
@@ -96,11 +97,14 @@ __device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, 
     const double kap = n.dR_d / n.dcv_d, cp = n.dcp_d, dTs = n.dTsr - n.dTmr;
     // ---- forward
     const double rf = 0.5 * (a.rho + b.rho);
-    const double pgm = 1.0 / rf, pgp = -pgm;
+    const double pgm = ca_drcp(rf), pgp = -pgm;
     const double dpa = kap * ((n.dT_0 * cp - (double)g.Phic[f - 1]) - a.K);
     const double dpb = kap * ((n.dT_0 * cp - (double)g.Phic[f]) - b.K);
-    const double thva = a.T / a.Pi, thvb = b.T / b.Pi;
-    const double z = (b.p - a.p) / a.p;
+    // reciprocals (v_rcp_f64 and two Newton-Raphson steps) in place of the
+    // correctly rounded float64 quotients
+    const double iPa = ca_drcp(a.Pi), iPib = ca_drcp(b.Pi), ipa = ca_drcp(a.p);
+    const double thva = a.T * iPa, thvb = b.T * iPib;
+    const double z = (b.p - a.p) * ipa;
     const double lr = n.dkex * log1p(z);
     const double Em = expm1(lr);
     const double dPi = a.Pi * Em;
@@ -116,10 +120,10 @@ __device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, 
     const double beta = n.rayleigh ? (double)g.aw_col[0] * (double)g.zeta_f[f] : 0.0;
     const double Pa7 = a.Pi * a.Pi * a.Pi * a.Pi * a.Pi * a.Pi * a.Pi;
     const double E7 = expm1(7.0 * lr);
-    const double Pb = a.Pi + dPi;
+    const double Pb = a.Pi + dPi, iPb = ca_drcp(Pb);
     const double Pb7 = Pb * Pb * Pb * Pb * Pb * Pb * Pb;
-    const double tha = (a.T - (n.dTmr + dTs * Pa7)) / a.Pi;
-    const double thb = (b.T - (n.dTmr + dTs * Pb7)) / Pb;
+    const double tha = (a.T - (n.dTmr + dTs * Pa7)) * iPa;
+    const double thb = (b.T - (n.dTmr + dTs * Pb7)) * iPb;
     // ---- reverse
     const double Lb = ob[0], Db = ob[1], Ub = ob[2], r0b = ob[3], r1b = ob[4], e0b = ob[5], e1b = ob[6], qb = ob[7];
     double x8b = 0.0, x9b = 0.0, x10b = 0.0;
@@ -130,17 +134,17 @@ __device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, 
     const double thab = -0.5 * cp * dPi * pgfb, thbb = thab;
     double dPib = -cp * 0.5 * (tha + thb) * pgfb;
     // thb = (T_b - T_r(Pb)) / Pb, tha likewise with Pa
-    double Tbb = thbb / Pb, Tab = thab / a.Pi;
-    double Pbb = -thbb * thb / Pb - thbb * dTs * 7.0 * (Pb7 / Pb) / Pb;
-    double Pab = -thab * tha / a.Pi;
-    double Pa7b = -dTs * thab / a.Pi;
+    double Tbb = thbb * iPb, Tab = thab * iPa;
+    double Pbb = -thbb * thb * iPb - thbb * dTs * 7.0 * (Pb7 * iPb) * iPb;
+    double Pab = -thab * tha * iPa;
+    double Pa7b = -dTs * thab * iPa;
     Pab += Pbb;
     dPib += Pbb;
     // dPhr = -cp (T_min lr + dTs / 7 Pa^7 expm1(7 lr))
     double lrb = -cp * n.dTmr * dPhrb;
-    Pa7b += -cp * dTs / 7.0 * E7 * dPhrb;
-    lrb += -cp * dTs / 7.0 * Pa7 * 7.0 * (E7 + 1.0) * dPhrb;
-    Pab += 7.0 * (Pa7 / a.Pi) * Pa7b;
+    Pa7b += -cp * dTs * (1.0 / 7.0) * E7 * dPhrb;
+    lrb += -cp * dTs * (1.0 / 7.0) * Pa7 * 7.0 * (E7 + 1.0) * dPhrb;
+    Pab += 7.0 * (Pa7 * iPa) * Pa7b;
     // up, d, lo
     double pgmb = 0.0, pgpb = 0.0, kab = 0.0, kbb = 0.0;
     pgpb += dtg * kb * 0.5 * gp * x10 * Ub;
@@ -171,14 +175,14 @@ __device__ __forceinline__ FaceVals face_vjp(const DevGeom &g, const DevNum &n, 
     // dPi = Pa expm1(lr), lr = kex log1p((pb - pa) / pa)
     Pab += Em * dPib;
     lrb += a.Pi * (Em + 1.0) * dPib;
-    const double zb = n.dkex * lrb / (1.0 + z);
-    double pbb = zb / a.p;
-    double pab = -zb * b.p / (a.p * a.p);
+    const double zb = n.dkex * lrb * ca_drcp(1.0 + z);
+    double pbb = zb * ipa;
+    double pab = -zb * b.p * (ipa * ipa);
     // thv = T / Pi
-    Tab += 0.5 * thbarb / a.Pi;
-    Pab -= 0.5 * thbarb * thva / a.Pi;
-    Tbb += 0.5 * thbarb / b.Pi;
-    double Pib_b = -0.5 * thbarb * thvb / b.Pi;
+    Tab += 0.5 * thbarb * iPa;
+    Pab -= 0.5 * thbarb * thva * iPa;
+    Tbb += 0.5 * thbarb * iPib;
+    double Pib_b = -0.5 * thbarb * thvb * iPib;
     // dp = kap ((T0 cp - Phi) - K)
     const double Kab = -kap * dpab, Kbb = -kap * dpbb;
     // pgp = -pgm, pgm = 1 / rf, rf = (rho_a + rho_b) / 2

@@ -235,7 +235,11 @@ ca_status ca_dss_fv(ca_handle *h, const ca_fieldvector *Y, double t, void *strea
  *                           2 = one wavefront per column, 32-column tiles (PCR);
  *                           1 and 2 need nlev + 1 <= 64.
  *   CA_TUNE_EXPLICIT_CHUNK: levels per work item of the explicit-tendency
- *                           kernel (0 = whole column).
+ *                           kernel (0 = whole column).  Setting it sets the
+ *                           chunk of ca_step's last explicit launch (which
+ *                           also forms the final combination) too; by
+ *                           default that launch has a chunk of its own, and
+ *                           ca_get_tuning reports the other launches'.
  *   CA_TUNE_DSS_CHUNK:      levels per work item of the DSS kernel (>= 1).
  *   CA_TUNE_DSS_KIND:       0 = staged through LDS per element cluster,
  *                           1 = direct gather from global memory.  Both sum
